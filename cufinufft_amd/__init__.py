@@ -1,0 +1,6 @@
+"""MI355X-native NUFFT (types 1 and 2, 1D/2D/3D, FP32/FP64) behind the
+cufinufft C ABI.  ``from cufinufft_amd import cufinufft`` is the drop-in for
+``from cufinufft import cufinufft``."""
+from cufinufft_amd.cufinufft import cufinufft  # noqa: F401
+
+__all__ = ["cufinufft"]

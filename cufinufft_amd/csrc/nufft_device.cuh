@@ -1,0 +1,702 @@
+// Device-side building blocks of the NUFFT path for gfx950: coordinate
+// rescaling, bin indices, ES-kernel evaluation, and every kernel of the
+// setpts / execute stages.  Templated on the plan precision T, the dimension
+// DIM and (for the spread/interp kernels) the kernel width NS, so that the
+// tap loops are fully unrolled for each width.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cfa {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cpx {
+    T x, y;
+};
+
+// 1/(2 pi) as the double literal the reference uses (contrib/utils.h:18).
+constexpr double kInv2Pi = 0.159154943091895336;
+
+// Geometry of the fine grid and its bins.  Unused axes have nf = bs = nb = 1.
+struct BinGeom {
+    int nf[3];
+    int bs[3];
+    int nb[3];
+};
+
+template <typename T>
+struct KerParams {
+    T beta;  // ES beta
+    T c;     // ES c = 4/ns^2
+    T hw;    // half width ns/2
+};
+
+// Subproblem bookkeeping written by setpts, read by the SM kernels.
+struct SubprobArgs {
+    const int *s2b;       // subproblem -> bin
+    const int *substart;  // first subproblem of each bin (nbins+1)
+    const int *binstart;  // first sorted point of each bin
+    const int *binsize;   // points per bin
+    const int *counters;  // [0] points, [1] subproblems
+    int maxsub;
+};
+
+// ---------------------------------------------------------------------------
+// RESCALE (pirange = 1), contrib/spreadinterp.h:36-38: x*M_1_2PI + off in
+// double (one fused multiply-add, as nvcc contracts it by default), times N in
+// double.  Returned unrounded: the caller rounds to T for the bin index (the
+// reference's value, bit for bit) and keeps the double for tap offsets.
+__device__ __forceinline__ double rescale_exact(float x, int n)
+{
+    const float pi = 3.14159265358979323846f;  // (float)M_PI
+    const double off = x < -pi ? 1.5 : (x >= pi ? -0.5 : 0.5);
+    return __builtin_fma((double)x, kInv2Pi, off) * (double)n;
+}
+__device__ __forceinline__ double rescale_exact(double x, int n)
+{
+    const double pi = 3.14159265358979323846;
+    const double off = x < -pi ? 1.5 : (x >= pi ? -0.5 : 0.5);
+    return __builtin_fma(x, kInv2Pi, off) * (double)n;
+}
+
+// Bin of one axis from the T-rounded rescaled coordinate:
+// b = floor(xr/bs) in T; b == nb -> nb-1; b < 0 -> 0 (spreadinterp2d.cu:
+// 113-118).  Written so that NaN or far out-of-range input (which the
+// reference indexes out of bounds with) lands in a valid bin instead.
+template <typename T>
+__device__ __forceinline__ int bin1(double v, int bs, int nb)
+{
+    const T xt = (T)v;
+    const T q = floor(xt / (T)bs);
+    return q >= (T)0 ? (q < (T)nb ? (int)q : nb - 1) : 0;
+}
+
+template <typename T, int DIM>
+__device__ __forceinline__ int point_bin(const T *__restrict__ x, const T *__restrict__ y,
+                                         const T *__restrict__ z, int i, const BinGeom &g)
+{
+    int b = bin1<T>(rescale_exact(x[i], g.nf[0]), g.bs[0], g.nb[0]);
+    if (DIM > 1) b += g.nb[0] * bin1<T>(rescale_exact(y[i], g.nf[1]), g.bs[1], g.nb[1]);
+    if (DIM > 2)
+        b += g.nb[0] * g.nb[1] * bin1<T>(rescale_exact(z[i], g.nf[2]), g.bs[2], g.nb[2]);
+    return b;
+}
+
+// ES kernel phi(x) = exp(beta sqrt(1 - c x^2)) for |x| < ns/2, else 0
+// (src/cuspreadinterp.h:6-16).  The argument of sqrt is floored at 0 so that
+// rounding at |x| -> ns/2 cannot produce NaN.
+__device__ __forceinline__ float es_eval(float x, const KerParams<float> &k)
+{
+    const float a = fmaxf(1.0f - k.c * x * x, 0.0f);
+    return fabsf(x) < k.hw ? expf(k.beta * sqrtf(a)) : 0.0f;
+}
+__device__ __forceinline__ double es_eval(double x, const KerParams<double> &k)
+{
+    const double a = fmax(1.0 - k.c * x * x, 0.0);
+    return fabs(x) < k.hw ? exp(k.beta * sqrt(a)) : 0.0;
+}
+
+// Position used for tap offsets: the unrounded rescale, clamped to
+// [-1, n+1] so that invalid input cannot index outside one period of wrap.
+__device__ __forceinline__ double tap_pos(double v, int n)
+{
+    return fmin(fmax(v, -1.0), (double)n + 1.0);
+}
+
+// First tap index and the ns kernel values of one axis:
+// start = ceil(v - ns/2) (exact in double), ker[i] = phi(start + i - v).
+template <typename T, int NS>
+__device__ __forceinline__ int taps_global(double v, const KerParams<T> &kp, T *ker)
+{
+    const int start = (int)ceil(v - 0.5 * NS);
+    const T x1 = (T)((double)start - v);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) ker[i] = es_eval(x1 + (T)i, kp);
+    return start;
+}
+
+// The same for a point inside a tile whose cell 0 sits at global index o and
+// which has width tw: the local first tap is clamped into [0, tw-ns] and the
+// kernel values are evaluated at the true distances of the clamped taps, so
+// a tap that falls outside the tile (possible only within rounding of the
+// kernel edge, where phi is 0) simply gets weight phi(.) of its real offset.
+template <typename T, int NS>
+__device__ __forceinline__ int taps_local(double v, int o, int tw, const KerParams<T> &kp,
+                                          T *ker)
+{
+    int l = (int)ceil(v - 0.5 * NS) - o;
+    l = l < 0 ? 0 : (l > tw - NS ? tw - NS : l);
+    const T x1 = (T)((double)(o + l) - v);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) ker[i] = es_eval(x1 + (T)i, kp);
+    return l;
+}
+
+__device__ __forceinline__ int wrap1(int i, int n)
+{
+    return i < 0 ? i + n : (i >= n ? i - n : i);
+}
+__device__ __forceinline__ int wrapmod(int i, int n)
+{
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+
+// Fire-and-forget device-scope floating-point atomic add to global memory
+// (global_atomic_add_f32 / _f64; the fine grid is coarse-grained hipMalloc
+// memory, where the hardware float atomics are exact).
+__device__ __forceinline__ void gadd(float *p, float v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void gadd(double *p, double v) { unsafeAtomicAdd(p, v); }
+
+// LDS atomic add (ds_add_f32 / ds_add_f64).
+__device__ __forceinline__ void ladd(float *p, float v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void ladd(double *p, double v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// ===========================================================================
+// setpts: counting bin sort
+// ===========================================================================
+
+// Per point: bin index and in-bin rank by atomic arrival
+// (CalcBinSize_noghost_{2,3}d, spreadinterp2d.cu:102-126).
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+k_bin_count(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
+            BinGeom g, int *__restrict__ binsize, int *__restrict__ rank,
+            int *__restrict__ binof)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+        const int b = point_bin<T, DIM>(x, y, z, i, g);
+        binof[i] = b;
+        rank[i] = atomicAdd(&binsize[b], 1);
+    }
+}
+
+// idxnupts[binstart[b] + rank] = i (CalcInvertofGlobalSortIdx, spreadinterp2d.cu:128-149).
+static __global__ void __launch_bounds__(256)
+k_bin_scatter(int M, const int *__restrict__ binof, const int *__restrict__ rank,
+              const int *__restrict__ binstart, int *__restrict__ idx)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x)
+        idx[binstart[binof[i]] + rank[i]] = i;
+}
+
+// Exclusive scans of (binsize, numsubprob = ceil(binsize/maxsub)) in three
+// passes: per-chunk sums, one-block scan of the chunk sums, per-chunk scan.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanChunk = kScanThreads * kScanItems;
+
+__device__ __forceinline__ int2 add2(int2 a, int2 b) { return make_int2(a.x + b.x, a.y + b.y); }
+
+// Block-wide exclusive scan of one int2 per thread; returns the block total.
+__device__ __forceinline__ int2 block_exscan2(int2 v, int2 &excl)
+{
+    __shared__ int2 wsum[kScanThreads / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int2 inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int2 o;
+        o.x = __shfl_up(inc.x, d, 64);
+        o.y = __shfl_up(inc.y, d, 64);
+        if (lane >= d) inc = add2(inc, o);
+    }
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    int2 off = make_int2(0, 0), tot = make_int2(0, 0);
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+        if (w < wid) off = add2(off, wsum[w]);
+        tot = add2(tot, wsum[w]);
+    }
+    excl = make_int2(off.x + inc.x - v.x, off.y + inc.y - v.y);
+    __syncthreads();
+    return tot;
+}
+
+__device__ __forceinline__ int2 load_pair(const int *__restrict__ cnt, long long i, long long n,
+                                          int maxsub)
+{
+    const int c = i < n ? cnt[i] : 0;
+    return make_int2(c, (c + maxsub - 1) / maxsub);
+}
+
+static __global__ void __launch_bounds__(kScanThreads)
+k_scan_reduce(const int *__restrict__ cnt, long long n, int maxsub, int2 *__restrict__ part)
+{
+    const long long base = (long long)blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
+    int2 s = make_int2(0, 0);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) s = add2(s, load_pair(cnt, base + k, n, maxsub));
+    int2 ex;
+    const int2 tot = block_exscan2(s, ex);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// One block: exclusive scan of the chunk sums in place; writes the totals to
+// counters[0..1] and substart[n].
+static __global__ void __launch_bounds__(kScanThreads)
+k_scan_parts(int2 *__restrict__ part, int nparts, int *__restrict__ counters,
+             int *__restrict__ substart_end)
+{
+    int2 carry = make_int2(0, 0);
+    for (int base = 0; base < nparts; base += kScanChunk) {
+        int2 v[kScanItems];
+        int2 s = make_int2(0, 0);
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const int i = base + threadIdx.x * kScanItems + k;
+            v[k] = i < nparts ? part[i] : make_int2(0, 0);
+            s = add2(s, v[k]);
+        }
+        int2 ex;
+        const int2 tot = block_exscan2(s, ex);
+        int2 run = add2(carry, ex);
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const int i = base + threadIdx.x * kScanItems + k;
+            if (i < nparts) part[i] = run;
+            run = add2(run, v[k]);
+        }
+        carry = add2(carry, tot);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        counters[0] = carry.x;
+        counters[1] = carry.y;
+        *substart_end = carry.y;
+    }
+}
+
+static __global__ void __launch_bounds__(kScanThreads)
+k_scan_apply(const int *__restrict__ cnt, long long n, int maxsub, const int2 *__restrict__ part,
+             int *__restrict__ binstart, int *__restrict__ substart)
+{
+    const long long base = (long long)blockIdx.x * kScanChunk + threadIdx.x * kScanItems;
+    int2 v[kScanItems];
+    int2 s = make_int2(0, 0);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = load_pair(cnt, base + k, n, maxsub);
+        s = add2(s, v[k]);
+    }
+    int2 ex;
+    block_exscan2(s, ex);
+    int2 run = add2(part[blockIdx.x], ex);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) {
+            binstart[base + k] = run.x;
+            substart[base + k] = run.y;
+        }
+        run = add2(run, v[k]);
+    }
+}
+
+// subprob_to_bin[substart[b] + j] = b for j < numsubprob[b]
+// (MapBintoSubProb, precision_independent.cu:75-85,117-127).
+static __global__ void __launch_bounds__(256)
+k_map_subprob(long long nbins, const int *__restrict__ substart, int *__restrict__ s2b)
+{
+    for (long long b = blockIdx.x * (long long)blockDim.x + threadIdx.x; b < nbins;
+         b += (long long)gridDim.x * blockDim.x) {
+        const int s0 = substart[b], s1 = substart[b + 1];
+        for (int s = s0; s < s1; ++s) s2b[s] = (int)b;
+    }
+}
+
+static __global__ void __launch_bounds__(256) k_fill_int(long long n, int v, int *__restrict__ p)
+{
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+static __global__ void __launch_bounds__(256) k_iota(int M, int *__restrict__ idx)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x)
+        idx[i] = i;
+}
+
+// ===========================================================================
+// execute: spreading (type 1) and interpolation (type 2)
+// ===========================================================================
+
+// GM / GM-sort spread: one thread per point, in idxnupts order, global float
+// atomics for each of the ns^DIM taps; all ntr transforms of the batch reuse
+// the kernel values (Spread_{2,3}d_NUptsdriven, spreadinterp2d.cu:16-58).
+template <typename T, int DIM, int NS>
+__global__ void __launch_bounds__(256)
+k_spread_gm(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
+            const int *__restrict__ idx, const cpx<T> *__restrict__ c, cpx<T> *__restrict__ fw,
+            BinGeom g, KerParams<T> kp, int ntr, long long nftot)
+{
+    const int n1 = g.nf[0], n2 = g.nf[1], n3 = g.nf[2];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+        const int j = idx[i];
+        T k1[NS], k2[NS], k3[NS];
+        const int s1 = taps_global<T, NS>(tap_pos(rescale_exact(x[j], n1), n1), kp, k1);
+        int s2 = 0, s3 = 0;
+        if (DIM > 1) s2 = taps_global<T, NS>(tap_pos(rescale_exact(y[j], n2), n2), kp, k2);
+        if (DIM > 2) s3 = taps_global<T, NS>(tap_pos(rescale_exact(z[j], n3), n3), kp, k3);
+        int ix[NS];
+#pragma unroll
+        for (int a = 0; a < NS; ++a) ix[a] = wrap1(s1 + a, n1);
+        for (int t = 0; t < ntr; ++t) {
+            const cpx<T> cv = c[(long long)t * M + j];
+            cpx<T> *f = fw + (long long)t * nftot;
+#pragma unroll
+            for (int dz = 0; dz < (DIM > 2 ? NS : 1); ++dz) {
+                const int iz = DIM > 2 ? wrap1(s3 + dz, n3) : 0;
+                const T kz = DIM > 2 ? k3[dz] : (T)1;
+#pragma unroll
+                for (int dy = 0; dy < (DIM > 1 ? NS : 1); ++dy) {
+                    const int iy = DIM > 1 ? wrap1(s2 + dy, n2) : 0;
+                    const T kyz = (DIM > 1 ? k2[dy] : (T)1) * kz;
+                    const T cr = cv.x * kyz, ci = cv.y * kyz;
+                    cpx<T> *row = f + ((long long)iz * n2 + iy) * n1;
+#pragma unroll
+                    for (int a = 0; a < NS; ++a) {
+                        gadd(&row[ix[a]].x, cr * k1[a]);
+                        gadd(&row[ix[a]].y, ci * k1[a]);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// GM / GM-sort interp: one thread per point gathers ns^DIM taps
+// (Interp_{2,3}d_NUptsdriven, spreadinterp2d.cu:516-554).
+template <typename T, int DIM, int NS>
+__global__ void __launch_bounds__(256)
+k_interp_gm(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
+            const int *__restrict__ idx, cpx<T> *__restrict__ c, const cpx<T> *__restrict__ fw,
+            BinGeom g, KerParams<T> kp, int ntr, long long nftot)
+{
+    const int n1 = g.nf[0], n2 = g.nf[1], n3 = g.nf[2];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+        const int j = idx[i];
+        T k1[NS], k2[NS], k3[NS];
+        const int s1 = taps_global<T, NS>(tap_pos(rescale_exact(x[j], n1), n1), kp, k1);
+        int s2 = 0, s3 = 0;
+        if (DIM > 1) s2 = taps_global<T, NS>(tap_pos(rescale_exact(y[j], n2), n2), kp, k2);
+        if (DIM > 2) s3 = taps_global<T, NS>(tap_pos(rescale_exact(z[j], n3), n3), kp, k3);
+        int ix[NS];
+#pragma unroll
+        for (int a = 0; a < NS; ++a) ix[a] = wrap1(s1 + a, n1);
+        for (int t = 0; t < ntr; ++t) {
+            const cpx<T> *f = fw + (long long)t * nftot;
+            T ar = 0, ai = 0;
+#pragma unroll
+            for (int dz = 0; dz < (DIM > 2 ? NS : 1); ++dz) {
+                const int iz = DIM > 2 ? wrap1(s3 + dz, n3) : 0;
+                const T kz = DIM > 2 ? k3[dz] : (T)1;
+#pragma unroll
+                for (int dy = 0; dy < (DIM > 1 ? NS : 1); ++dy) {
+                    const int iy = DIM > 1 ? wrap1(s2 + dy, n2) : 0;
+                    const T kyz = (DIM > 1 ? k2[dy] : (T)1) * kz;
+                    const cpx<T> *row = f + ((long long)iz * n2 + iy) * n1;
+                    T rr = 0, ri = 0;
+#pragma unroll
+                    for (int a = 0; a < NS; ++a) {
+                        const cpx<T> v = row[ix[a]];
+                        rr += v.x * k1[a];
+                        ri += v.y * k1[a];
+                    }
+                    ar += rr * kyz;
+                    ai += ri * kyz;
+                }
+            }
+            cpx<T> o;
+            o.x = ar;
+            o.y = ai;
+            c[(long long)t * M + j] = o;
+        }
+    }
+}
+
+// Tile of one subproblem: the bin's cells plus a halo of P = ceil(ns/2) on
+// each side; cell 0 of the tile is global index o[d] = bin*bs - P.
+template <int DIM, int NS>
+struct Tile {
+    static constexpr int P = (NS + 1) / 2;
+    int tw[3];
+    int o[3];
+    int n;
+    __device__ __forceinline__ Tile(const BinGeom &g, int b)
+    {
+        const int bx = b % g.nb[0];
+        const int by = (b / g.nb[0]) % g.nb[1];
+        const int bz = b / (g.nb[0] * g.nb[1]);
+        tw[0] = g.bs[0] + 2 * P;
+        tw[1] = DIM > 1 ? g.bs[1] + 2 * P : 1;
+        tw[2] = DIM > 2 ? g.bs[2] + 2 * P : 1;
+        o[0] = bx * g.bs[0] - P;
+        o[1] = DIM > 1 ? by * g.bs[1] - P : 0;
+        o[2] = DIM > 2 ? bz * g.bs[2] - P : 0;
+        n = tw[0] * tw[1] * tw[2];
+    }
+};
+
+// SM spread (Spread_{2,3}d_Subprob, spreadinterp2d.cu:152-231,
+// spreadinterp3d.cu:285-389): a workgroup takes one (subproblem, transform)
+// job at a time from a grid-stride queue, zeroes an LDS tile (separate
+// re / im planes, so consecutive x taps hit consecutive banks), accumulates
+// every tap of the subproblem's points with LDS float atomics, then flushes
+// the non-zero tile cells to the fine grid with global float atomics (the
+// halo overlaps neighbouring bins).  The job count comes from device memory,
+// so setpts never has to copy it back to the host.
+template <typename T, int DIM, int NS>
+__global__ void __launch_bounds__(256)
+k_spread_sm(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
+            const int *__restrict__ idx, const cpx<T> *__restrict__ c, cpx<T> *__restrict__ fw,
+            BinGeom g, KerParams<T> kp, SubprobArgs sa, int ntr, long long nftot)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int total = sa.counters[1];
+    const long long njobs = (long long)total * ntr;
+    const int n1 = g.nf[0], n2 = g.nf[1], n3 = g.nf[2];
+    for (long long job = blockIdx.x; job < njobs; job += gridDim.x) {
+        const int t = (int)(job / total);
+        const int s = (int)(job - (long long)t * total);
+        const int b = sa.s2b[s];
+        const int kk = s - sa.substart[b];
+        const int p0 = sa.binstart[b] + kk * sa.maxsub;
+        const int np = min(sa.maxsub, sa.binsize[b] - kk * sa.maxsub);
+        const Tile<DIM, NS> tl(g, b);
+        T *sre = reinterpret_cast<T *>(smem);
+        T *sim = sre + tl.n;
+        for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
+            sre[q] = 0;
+            sim[q] = 0;
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < np; q += blockDim.x) {
+            const int j = idx[p0 + q];
+            T k1[NS], k2[NS], k3[NS];
+            const int l1 = taps_local<T, NS>(tap_pos(rescale_exact(x[j], n1), n1), tl.o[0],
+                                             tl.tw[0], kp, k1);
+            int l2 = 0, l3 = 0;
+            if (DIM > 1)
+                l2 = taps_local<T, NS>(tap_pos(rescale_exact(y[j], n2), n2), tl.o[1], tl.tw[1],
+                                       kp, k2);
+            if (DIM > 2)
+                l3 = taps_local<T, NS>(tap_pos(rescale_exact(z[j], n3), n3), tl.o[2], tl.tw[2],
+                                       kp, k3);
+            const cpx<T> cv = c[(long long)t * M + j];
+#pragma unroll
+            for (int dz = 0; dz < (DIM > 2 ? NS : 1); ++dz) {
+                const T kz = DIM > 2 ? k3[dz] : (T)1;
+#pragma unroll
+                for (int dy = 0; dy < (DIM > 1 ? NS : 1); ++dy) {
+                    const T kyz = (DIM > 1 ? k2[dy] : (T)1) * kz;
+                    const T cr = cv.x * kyz, ci = cv.y * kyz;
+                    const int row = ((l3 + dz) * tl.tw[1] + (l2 + dy)) * tl.tw[0] + l1;
+#pragma unroll
+                    for (int a = 0; a < NS; ++a) {
+                        ladd(&sre[row + a], cr * k1[a]);
+                        ladd(&sim[row + a], ci * k1[a]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        cpx<T> *f = fw + (long long)t * nftot;
+        for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
+            const T vr = sre[q], vi = sim[q];
+            if (vr != (T)0 || vi != (T)0) {
+                const int qx = q % tl.tw[0];
+                const int qy = (q / tl.tw[0]) % tl.tw[1];
+                const int qz = q / (tl.tw[0] * tl.tw[1]);
+                const int gx = wrapmod(tl.o[0] + qx, n1);
+                const int gy = DIM > 1 ? wrapmod(tl.o[1] + qy, n2) : 0;
+                const int gz = DIM > 2 ? wrapmod(tl.o[2] + qz, n3) : 0;
+                cpx<T> *p = f + ((long long)gz * n2 + gy) * n1 + gx;
+                gadd(&p->x, vr);
+                gadd(&p->y, vi);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// SM interp (Interp_{2,3}d_Subprob, spreadinterp2d.cu:596-669): load the
+// padded tile of the subproblem's bin into LDS once, then each thread gathers
+// the taps of its points from LDS.
+template <typename T, int DIM, int NS>
+__global__ void __launch_bounds__(256)
+k_interp_sm(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
+            const int *__restrict__ idx, cpx<T> *__restrict__ c, const cpx<T> *__restrict__ fw,
+            BinGeom g, KerParams<T> kp, SubprobArgs sa, int ntr, long long nftot)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int total = sa.counters[1];
+    const long long njobs = (long long)total * ntr;
+    const int n1 = g.nf[0], n2 = g.nf[1], n3 = g.nf[2];
+    for (long long job = blockIdx.x; job < njobs; job += gridDim.x) {
+        const int t = (int)(job / total);
+        const int s = (int)(job - (long long)t * total);
+        const int b = sa.s2b[s];
+        const int kk = s - sa.substart[b];
+        const int p0 = sa.binstart[b] + kk * sa.maxsub;
+        const int np = min(sa.maxsub, sa.binsize[b] - kk * sa.maxsub);
+        const Tile<DIM, NS> tl(g, b);
+        T *sre = reinterpret_cast<T *>(smem);
+        T *sim = sre + tl.n;
+        const cpx<T> *f = fw + (long long)t * nftot;
+        for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
+            const int qx = q % tl.tw[0];
+            const int qy = (q / tl.tw[0]) % tl.tw[1];
+            const int qz = q / (tl.tw[0] * tl.tw[1]);
+            const int gx = wrapmod(tl.o[0] + qx, n1);
+            const int gy = DIM > 1 ? wrapmod(tl.o[1] + qy, n2) : 0;
+            const int gz = DIM > 2 ? wrapmod(tl.o[2] + qz, n3) : 0;
+            const cpx<T> v = f[((long long)gz * n2 + gy) * n1 + gx];
+            sre[q] = v.x;
+            sim[q] = v.y;
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < np; q += blockDim.x) {
+            const int j = idx[p0 + q];
+            T k1[NS], k2[NS], k3[NS];
+            const int l1 = taps_local<T, NS>(tap_pos(rescale_exact(x[j], n1), n1), tl.o[0],
+                                             tl.tw[0], kp, k1);
+            int l2 = 0, l3 = 0;
+            if (DIM > 1)
+                l2 = taps_local<T, NS>(tap_pos(rescale_exact(y[j], n2), n2), tl.o[1], tl.tw[1],
+                                       kp, k2);
+            if (DIM > 2)
+                l3 = taps_local<T, NS>(tap_pos(rescale_exact(z[j], n3), n3), tl.o[2], tl.tw[2],
+                                       kp, k3);
+            T ar = 0, ai = 0;
+#pragma unroll
+            for (int dz = 0; dz < (DIM > 2 ? NS : 1); ++dz) {
+                const T kz = DIM > 2 ? k3[dz] : (T)1;
+#pragma unroll
+                for (int dy = 0; dy < (DIM > 1 ? NS : 1); ++dy) {
+                    const T kyz = (DIM > 1 ? k2[dy] : (T)1) * kz;
+                    const int row = ((l3 + dz) * tl.tw[1] + (l2 + dy)) * tl.tw[0] + l1;
+                    T rr = 0, ri = 0;
+#pragma unroll
+                    for (int a = 0; a < NS; ++a) {
+                        rr += sre[row + a] * k1[a];
+                        ri += sim[row + a] * k1[a];
+                    }
+                    ar += rr * kyz;
+                    ai += ri * kyz;
+                }
+            }
+            cpx<T> o;
+            o.x = ar;
+            o.y = ai;
+            c[(long long)t * M + j] = o;
+        }
+        __syncthreads();
+    }
+}
+
+// ===========================================================================
+// execute: deconvolve (type 1) / amplify (type 2), one fused pass each
+// ===========================================================================
+
+// Frequency of grid index l along an axis with ms modes and nf cells, or
+// INT_MIN if l holds no mode (inverse of the w(k) map of deconvolve_wrapper.cu:32-35).
+__device__ __forceinline__ int grid2freq(int l, int ms, int nf)
+{
+    if (l <= (ms - 1) / 2) return l;
+    if (l >= nf - ms / 2) return l - nf;
+    return INT32_MIN;
+}
+__device__ __forceinline__ int freq2grid(int f, int nf) { return f >= 0 ? f : nf + f; }
+
+// fk[t][k] = fw[t][w(k)] / prod_d fwkerhalf_d[|k_d - ms_d/2|]
+// (Deconvolve_{2,3}d, deconvolve_wrapper.cu:26-66).  One workgroup per output
+// row (fixed k2, k3, t); threads sweep k1 so both reads and writes coalesce.
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+k_deconvolve(int ms, int mt, int mu, int n1, int n2, int n3, const cpx<T> *__restrict__ fw,
+             cpx<T> *__restrict__ fk, const T *__restrict__ ker1, const T *__restrict__ ker2,
+             const T *__restrict__ ker3, long long rows, long long nftot)
+{
+    for (long long r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int k2 = DIM > 1 ? (int)(r % mt) : 0;
+        const int k3 = DIM > 2 ? (int)((r / mt) % mu) : 0;
+        const long long t = r / ((long long)mt * mu);
+        const int f2 = k2 - mt / 2, f3 = k3 - mu / 2;
+        const T kyz = (DIM > 1 ? ker2[abs(f2)] : (T)1) * (DIM > 2 ? ker3[abs(f3)] : (T)1);
+        const cpx<T> *src = fw + t * nftot +
+                            ((long long)(DIM > 2 ? freq2grid(f3, n3) : 0) * n2 +
+                             (DIM > 1 ? freq2grid(f2, n2) : 0)) *
+                                n1;
+        cpx<T> *dst = fk + r * ms;
+        for (int k1 = threadIdx.x; k1 < ms; k1 += blockDim.x) {
+            const int f1 = k1 - ms / 2;
+            const T kv = ker1[abs(f1)] * kyz;
+            const cpx<T> v = src[freq2grid(f1, n1)];
+            cpx<T> o;
+            o.x = v.x / kv;
+            o.y = v.y / kv;
+            dst[k1] = o;
+        }
+    }
+}
+
+// Type-2 step 1 as one pass over the fine grid: every cell is written once,
+// with fk[k]/prod fwkerhalf at mode cells and 0 elsewhere (Amplify_{2,3}d plus
+// the memset before it, deconvolve_wrapper.cu:81-121,181-182).
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+k_amplify(int ms, int mt, int mu, int n1, int n2, int n3, cpx<T> *__restrict__ fw,
+          const cpx<T> *__restrict__ fk, const T *__restrict__ ker1, const T *__restrict__ ker2,
+          const T *__restrict__ ker3, long long rows, long long ntot)
+{
+    for (long long r = blockIdx.x; r < rows; r += gridDim.x) {
+        const int l2 = DIM > 1 ? (int)(r % n2) : 0;
+        const int l3 = DIM > 2 ? (int)((r / n2) % n3) : 0;
+        const long long t = r / ((long long)n2 * n3);
+        const int f2 = DIM > 1 ? grid2freq(l2, mt, n2) : 0;
+        const int f3 = DIM > 2 ? grid2freq(l3, mu, n3) : 0;
+        cpx<T> *dst = fw + r * n1;
+        cpx<T> zero;
+        zero.x = 0;
+        zero.y = 0;
+        if (f2 == INT32_MIN || f3 == INT32_MIN) {
+            for (int l1 = threadIdx.x; l1 < n1; l1 += blockDim.x) dst[l1] = zero;
+            continue;
+        }
+        const T kyz = (DIM > 1 ? ker2[abs(f2)] : (T)1) * (DIM > 2 ? ker3[abs(f3)] : (T)1);
+        const cpx<T> *src =
+            fk + t * ntot + ((long long)(f3 + mu / 2) * mt + (f2 + mt / 2)) * (long long)ms;
+        for (int l1 = threadIdx.x; l1 < n1; l1 += blockDim.x) {
+            const int f1 = grid2freq(l1, ms, n1);
+            cpx<T> o = zero;
+            if (f1 != INT32_MIN) {
+                const T kv = ker1[abs(f1)] * kyz;
+                const cpx<T> v = src[f1 + ms / 2];
+                o.x = v.x / kv;
+                o.y = v.y / kv;
+            }
+            dst[l1] = o;
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_zero(cpx<T> *__restrict__ p, long long n)
+{
+    cpx<T> zero;
+    zero.x = 0;
+    zero.y = 0;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = zero;
+}
+
+}  // namespace cfa

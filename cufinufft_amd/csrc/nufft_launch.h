@@ -1,0 +1,78 @@
+// Host-side launch interface between the plan logic (plan.cpp) and the HIP
+// kernels (nufft_device.cuh).  Every function returns 0 or a library error
+// code and launches asynchronously on the given stream.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace cfa {
+
+struct BinGeom;
+template <typename T> struct KerParams;
+template <typename T> struct cpx;
+
+constexpr int kErrHip = 10;
+constexpr int kErrFft = 11;
+constexpr int kErrLds = 12;
+constexpr size_t kMaxLdsBytes = 160 * 1024;  // LDS of one gfx950 workgroup
+
+// Device arrays written by setpts.
+struct SortBufs {
+    int *binsize = nullptr;   // [nbins]
+    int *binstart = nullptr;  // [nbins]
+    int *substart = nullptr;  // [nbins + 1]
+    int *s2b = nullptr;       // [nbins + ceil(M / maxsub)] upper bound
+    int *rank = nullptr;      // [M]
+    int *binof = nullptr;     // [M]
+    int *idx = nullptr;       // [M]
+    int *counters = nullptr;  // [2]: points, subproblems
+    void *scan_part = nullptr;  // int2[ceil(nbins / chunk)]
+};
+
+struct Launch {
+    hipStream_t stream = nullptr;
+    int ncu = 256;
+};
+
+// Bin sort of M points (counts, ranks, scans, scatter, subproblem map).
+template <typename T>
+int run_bin_sort(int dim, const BinGeom &g, long long nbins, int M, const T *x, const T *y,
+                 const T *z, int maxsub, bool need_subprob, SortBufs &sb, const Launch &L);
+// Identity order (gpu_sort = 0).
+int run_iota(int M, int *idx, const Launch &L);
+
+// Spread / interp of ntr transforms.  method 1 = GM(-sort), 2 = SM.
+template <typename T>
+int run_spread(int method, int dim, int ns, int M, const T *x, const T *y, const T *z,
+               const cpx<T> *c, cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp,
+               const SortBufs &sb, int maxsub, long long sub_bound, int ntr, long long nftot,
+               const Launch &L);
+template <typename T>
+int run_interp(int method, int dim, int ns, int M, const T *x, const T *y, const T *z,
+               cpx<T> *c, const cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp,
+               const SortBufs &sb, int maxsub, long long sub_bound, int ntr, long long nftot,
+               const Launch &L);
+
+// Bytes of LDS the SM kernels need for a tile (re and im planes).
+size_t sm_tile_bytes(int dim, int ns, const int *bs, size_t elem);
+
+template <typename T>
+int run_deconvolve(int dim, const int *ms, const int *nf, const cpx<T> *fw, cpx<T> *fk,
+                   const T *const *ker, int ntr, const Launch &L);
+template <typename T>
+int run_amplify(int dim, const int *ms, const int *nf, cpx<T> *fw, const cpx<T> *fk,
+                const T *const *ker, int ntr, const Launch &L);
+template <typename T>
+int run_zero(cpx<T> *p, long long n, const Launch &L);
+
+// Per-(T, DIM) spread/interp launchers (one translation unit each).
+template <typename T, int DIM>
+int spread_dim(int method, int ns, int M, const T *x, const T *y, const T *z, const cpx<T> *c,
+               cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp, const SortBufs &sb,
+               int maxsub, long long sub_bound, int ntr, long long nftot, const Launch &L);
+template <typename T, int DIM>
+int interp_dim(int method, int ns, int M, const T *x, const T *y, const T *z, cpx<T> *c,
+               const cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp, const SortBufs &sb,
+               int maxsub, long long sub_bound, int ntr, long long nftot, const Launch &L);
+
+}  // namespace cfa

@@ -1,0 +1,735 @@
+// Plan lifecycle and C ABI of the MI355X NUFFT library: makeplan / setpts /
+// execute / destroy for single and double precision (include/cufinufft.h).
+//
+// Host responsibilities: spreader parameters (in the plan precision, as the
+// reference's setup_spreader computes them), fine-grid sizes, the Fourier
+// series of the ES kernel (FP64 Gauss-Legendre quadrature), device buffers,
+// the hipFFT plans and the stage sequence of each execute.  All device work
+// is asynchronous on the plan's stream.
+#include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/cufinufft.h"
+#include "nufft_device.cuh"
+#include "nufft_launch.h"
+
+namespace cfa {
+
+// Error codes of contrib/utils.h:28-36.
+constexpr int kWarnEpsTooSmall = 1;
+constexpr int kErrSpreadBoxSmall = 3;
+constexpr int kErrUpsampTooSmall = 7;
+constexpr int kErrHornerWrongBeta = 8;
+constexpr int kErrNdataNotValid = 9;
+constexpr int kMaxNspread = 16;
+
+// ---------------------------------------------------------------------------
+// Spreader parameters, contrib/spreadinterp.cpp:6-67, evaluated in the plan
+// precision T exactly as the reference's overloads resolve (the single
+// precision build calls the float log10 / ceil / sqrt).
+template <typename T> struct Spread {
+    int ns = 0;
+    T beta = 0, c = 0, hw = 0;
+    T upsampfac = 2;
+};
+
+inline float m_log10(float v) { return std::log10(v); }
+inline double m_log10(double v) { return std::log10(v); }
+inline float m_log(float v) { return std::log(v); }
+inline double m_log(double v) { return std::log(v); }
+inline float m_sqrt(float v) { return std::sqrt(v); }
+inline double m_sqrt(double v) { return std::sqrt(v); }
+inline float m_ceil(float v) { return std::ceil(v); }
+inline double m_ceil(double v) { return std::ceil(v); }
+
+template <typename T> constexpr T eps_mach();
+template <> constexpr float eps_mach<float>() { return (float)6e-08; }   // contrib/utils_fp.h:34
+template <> constexpr double eps_mach<double>() { return 1.1e-16; }     // contrib/utils_fp.h:43
+
+template <typename T>
+int setup_spreader(Spread<T> &sp, T eps, T upsampfac, int kerevalmeth)
+{
+    const T pi = (T)M_PI;
+    if (upsampfac != 2.0) {
+        if (kerevalmeth == 1) return kErrHornerWrongBeta;
+        if (upsampfac <= 1.0) return kErrUpsampTooSmall;
+    }
+    int ier = 0;
+    if (eps < eps_mach<T>()) {
+        eps = eps_mach<T>();
+        ier = kWarnEpsTooSmall;
+    }
+    int ns = (int)m_ceil(-m_log10(eps / (T)10.0));
+    if (upsampfac != 2.0) ns = (int)m_ceil(-m_log(eps) / (pi * m_sqrt(1 - 1 / upsampfac)));
+    ns = std::max(2, ns);
+    if (ns > kMaxNspread) {
+        ns = kMaxNspread;
+        ier = kWarnEpsTooSmall;
+    }
+    sp.ns = ns;
+    sp.hw = (T)ns / 2;
+    sp.c = (T)(4.0 / (double)(T)(ns * ns));
+    T betaoverns = (T)2.30;
+    if (ns == 2) betaoverns = (T)2.20;
+    if (ns == 3) betaoverns = (T)2.26;
+    if (ns == 4) betaoverns = (T)2.38;
+    if (upsampfac != 2.0) {
+        const T gamma = (T)0.97;
+        betaoverns = gamma * pi * (1 - 1 / (2 * upsampfac));
+    }
+    sp.beta = betaoverns * (T)ns;
+    sp.upsampfac = upsampfac;
+    return ier;
+}
+
+// Smallest even 2,3,5-smooth integer >= n (contrib/utils.cpp:3-22).
+int next235even(int n)
+{
+    if (n <= 2) return 2;
+    if (n % 2) ++n;
+    for (int m = n;; m += 2) {
+        int r = m;
+        while (r % 2 == 0) r /= 2;
+        while (r % 3 == 0) r /= 3;
+        while (r % 5 == 0) r /= 5;
+        if (r == 1) return m;
+    }
+}
+
+// Fine-grid size of one axis (SET_NF_TYPE12, contrib/common.cpp:24-37).
+int fine_size(int ms, double upsampfac, int ns)
+{
+    long long nf = (long long)(upsampfac * ms);
+    if (nf < 2 * ns) nf = 2 * ns;
+    if (nf < 2147483647LL) nf = next235even((int)nf);
+    return (int)nf;
+}
+
+// ---------------------------------------------------------------------------
+// Fourier series of the ES kernel on a grid of nf cells, k = 0..nf/2
+// (onedim_fseries_kernel, contrib/common.cpp:39-124): the kernel is even and
+// supported on |z| < ns/2, so its transform is 2 * integral over [0, ns/2] of
+// phi(z) cos(2 pi k z / nf) dz, evaluated by q = (int)(2 + 1.5 ns)-point
+// Gauss-Legendre quadrature on the half support; the factor (-1)^k is the
+// phase of the nf/2 grid offset at which RESCALE places x = 0.  Computed in
+// FP64 with phasor recursion per node.
+static void gauss_legendre(int n, std::vector<double> &x, std::vector<double> &w)
+{
+    x.assign(n, 0.0);
+    w.assign(n, 0.0);
+    for (int i = 0; i < (n + 1) / 2; ++i) {
+        double z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = z;
+            for (int k = 2; k <= n; ++k) {
+                const double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            const double dp = n * (z * p1 - p0) / (z * z - 1.0);
+            const double dz = p1 / dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-16) break;
+        }
+        double p0 = 1.0, p1 = z;
+        for (int k = 2; k <= n; ++k) {
+            const double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        const double dp = n * (z * p1 - p0) / (z * z - 1.0);
+        x[i] = -z;
+        x[n - 1 - i] = z;
+        w[i] = w[n - 1 - i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    }
+}
+
+template <typename T>
+static void fseries(int nf, const Spread<T> &sp, std::vector<double> &out)
+{
+    const double J2 = sp.ns / 2.0;
+    const int q = (int)(2 + 3.0 * J2);
+    std::vector<double> z, w;
+    gauss_legendre(2 * q, z, w);
+    const double beta = (double)sp.beta, c = (double)sp.c;
+    std::vector<double> f(q), cs(q), sn(q), cr(q), ci(q);
+    for (int n = 0; n < q; ++n) {
+        const double zn = J2 * z[q + n];  // positive half of the 2q-node rule
+        const double a = 1.0 - c * zn * zn;
+        f[n] = J2 * w[q + n] * (std::fabs(zn) < J2 ? std::exp(beta * std::sqrt(a > 0 ? a : 0)) : 0);
+        const double th = 2.0 * M_PI * (nf / 2 - zn) / nf;
+        cs[n] = std::cos(th);
+        sn[n] = std::sin(th);
+        cr[n] = 1.0;
+        ci[n] = 0.0;
+    }
+    out.assign(nf / 2 + 1, 0.0);
+    for (int k = 0; k <= nf / 2; ++k) {
+        double s = 0;
+        for (int n = 0; n < q; ++n) {
+            s += 2.0 * f[n] * cr[n];
+            const double r = cr[n] * cs[n] - ci[n] * sn[n];
+            ci[n] = cr[n] * sn[n] + ci[n] * cs[n];
+            cr[n] = r;
+        }
+        out[k] = s;
+        if ((k & 63) == 63) {  // re-anchor the phasors to bound drift
+            for (int n = 0; n < q; ++n) {
+                const double zn = J2 * z[q + n];
+                const double th = 2.0 * M_PI * (double)(k + 1) * (nf / 2 - zn) / nf;
+                cr[n] = std::cos(th);
+                ci[n] = std::sin(th);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0) hipSetDevice(prev);
+    }
+};
+
+template <typename T>
+static int dmalloc(T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void **)p, n * sizeof(T)) == hipSuccess ? 0 : kErrHip;
+}
+template <typename T>
+static void dfree(T *&p)
+{
+    if (p) hipFree((void *)p);
+    p = nullptr;
+}
+
+template <typename T> struct FftType;
+template <> struct FftType<float> {
+    static constexpr hipfftType type = HIPFFT_C2C;
+    static hipfftResult exec(hipfftHandle h, cpx<float> *d, int dir)
+    {
+        return hipfftExecC2C(h, (hipfftComplex *)d, (hipfftComplex *)d, dir);
+    }
+};
+template <> struct FftType<double> {
+    static constexpr hipfftType type = HIPFFT_Z2Z;
+    static hipfftResult exec(hipfftHandle h, cpx<double> *d, int dir)
+    {
+        return hipfftExecZ2Z(h, (hipfftDoubleComplex *)d, (hipfftDoubleComplex *)d, dir);
+    }
+};
+
+template <typename T>
+struct Plan {
+    cufinufft_opts opts{};
+    Spread<T> sp;
+    int type = 0, dim = 0;
+    int ms[3] = {1, 1, 1};
+    int nf[3] = {1, 1, 1};
+    int ntransf = 1, maxbatch = 1, iflag = 1;
+    int method = 1;
+    BinGeom geom{};
+    long long nbins = 1;
+    long long nftot = 1, ntot = 1;
+
+    // points
+    int M = 0;
+    const T *kx = nullptr, *ky = nullptr, *kz = nullptr;
+    bool pts_set = false;
+    int cap_M = -1;
+    long long cap_sub = -1;
+    long long sub_bound = 0;
+
+    // device buffers
+    cpx<T> *fw = nullptr;
+    T *fwker[3] = {nullptr, nullptr, nullptr};
+    std::vector<double> fser[3];
+    SortBufs sb;
+
+    hipfftHandle fft_full = 0, fft_rem = 0;
+    int fft_rem_batch = 0;
+    bool fft_full_ok = false, fft_rem_ok = false;
+    Launch L;
+
+    ~Plan() { release(); }
+
+    void release()
+    {
+        DeviceGuard dg(opts.gpu_device_id);
+        if (fft_full_ok) hipfftDestroy(fft_full);
+        if (fft_rem_ok) hipfftDestroy(fft_rem);
+        fft_full_ok = fft_rem_ok = false;
+        dfree(fw);
+        for (auto &k : fwker) dfree(k);
+        dfree(sb.binsize);
+        dfree(sb.binstart);
+        dfree(sb.substart);
+        dfree(sb.s2b);
+        dfree(sb.rank);
+        dfree(sb.binof);
+        dfree(sb.idx);
+        dfree(sb.counters);
+        if (sb.scan_part) hipFree(sb.scan_part);
+        sb.scan_part = nullptr;
+    }
+
+    int make_fft(hipfftHandle *h, int batch)
+    {
+        int n[3];
+        for (int d = 0; d < dim; ++d) n[d] = nf[dim - 1 - d];  // slowest first
+        if (hipfftCreate(h) != HIPFFT_SUCCESS) return kErrFft;
+        if (hipfftPlanMany(h, dim, n, n, 1, (int)nftot, n, 1, (int)nftot, FftType<T>::type,
+                           batch) != HIPFFT_SUCCESS) {
+            hipfftDestroy(*h);
+            return kErrFft;
+        }
+        return 0;
+    }
+
+    int fft(cpx<T> *data, int batch)
+    {
+        hipfftHandle h;
+        if (batch == maxbatch) {
+            h = fft_full;
+        } else {
+            if (!fft_rem_ok || fft_rem_batch != batch) {
+                if (fft_rem_ok) hipfftDestroy(fft_rem);
+                fft_rem_ok = false;
+                int e = make_fft(&fft_rem, batch);
+                if (e) return e;
+                fft_rem_ok = true;
+                fft_rem_batch = batch;
+            }
+            h = fft_rem;
+        }
+        if (hipfftSetStream(h, L.stream) != HIPFFT_SUCCESS) return kErrFft;
+        return FftType<T>::exec(h, data, iflag) == HIPFFT_SUCCESS ? 0 : kErrFft;
+    }
+
+    int makeplan(int type_, int dim_, const int *nmodes, int iflag_, int ntransf_, T tol,
+                 int maxbatch_, const cufinufft_opts *o);
+    int setpts(int M_, const T *x, const T *y, const T *z);
+    int execute(cpx<T> *c, cpx<T> *fk);
+};
+
+int default_opts(int type, int dim, cufinufft_opts *opts)
+{
+    if (!opts) return 1;
+    opts->upsampfac = 2.0;
+    opts->gpu_nstreams = 0;
+    opts->gpu_sort = 1;
+    opts->gpu_maxsubprobsize = 1024;
+    opts->gpu_obinsizex = opts->gpu_obinsizey = opts->gpu_obinsizez = -1;
+    opts->gpu_binsizex = opts->gpu_binsizey = opts->gpu_binsizez = -1;
+    opts->gpu_spreadinterponly = 0;
+    opts->gpu_kerevalmeth = 0;
+    opts->gpu_device_id = 0;
+    if (dim < 1 || dim > 3) return 1;
+    if (type == 1)
+        opts->gpu_method = 2;
+    else if (type == 2)
+        opts->gpu_method = 1;
+    else
+        return 1;  // type 3 is not part of this library (nor of the reference)
+    return 0;
+}
+
+// Default bin sizes (SETUP_BINSIZE, src/cufinufft.cu:17-73).
+static void default_binsizes(int dim, cufinufft_opts &o)
+{
+    if (dim == 1) {
+        if (o.gpu_binsizex < 0) o.gpu_binsizex = 1024;
+        o.gpu_binsizey = o.gpu_binsizez = 1;
+    } else if (dim == 2) {
+        if (o.gpu_binsizex < 0) o.gpu_binsizex = 32;
+        if (o.gpu_binsizey < 0) o.gpu_binsizey = 32;
+        o.gpu_binsizez = 1;
+    } else {
+        if (o.gpu_binsizex < 0) o.gpu_binsizex = 16;
+        if (o.gpu_binsizey < 0) o.gpu_binsizey = 16;
+        if (o.gpu_binsizez < 0) o.gpu_binsizez = 2;
+    }
+}
+
+template <typename T>
+int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int ntransf_, T tol,
+                      int maxbatch_, const cufinufft_opts *o)
+{
+    if (type_ != 1 && type_ != 2) return 1;
+    if (dim_ < 1 || dim_ > 3) return 1;
+    if (!nmodes) return kErrNdataNotValid;
+    if (o) {
+        opts = *o;
+    } else {
+        int e = default_opts(type_, dim_, &opts);
+        if (e) return e;
+    }
+    type = type_;
+    dim = dim_;
+    for (int d = 0; d < 3; ++d) ms[d] = d < dim ? nmodes[d] : 1;
+    for (int d = 0; d < dim; ++d)
+        if (ms[d] < 1) return kErrNdataNotValid;
+    if (ntransf_ < 1) return kErrNdataNotValid;
+    if (opts.gpu_method < 0 || opts.gpu_method > 2) return 2;
+    if (opts.gpu_maxsubprobsize < 1) return 1;
+    int ier = setup_spreader<T>(sp, tol, (T)opts.upsampfac, opts.gpu_kerevalmeth);
+    if (ier > 1) return ier;
+    if (opts.gpu_kerevalmeth != 0 && opts.gpu_kerevalmeth != 1) return 1;
+
+    default_binsizes(dim, opts);
+    if (opts.gpu_binsizex < 1 || opts.gpu_binsizey < 1 || opts.gpu_binsizez < 1) return 1;
+    iflag = iflag_ >= 0 ? 1 : -1;
+    ntransf = ntransf_;
+    maxbatch = maxbatch_ == 0 ? std::min(ntransf, 8) : maxbatch_;
+    if (maxbatch < 1) return kErrNdataNotValid;
+    maxbatch = std::min(maxbatch, ntransf);
+
+    for (int d = 0; d < dim; ++d) {
+        if (opts.gpu_spreadinterponly) {
+            nf[d] = ms[d];
+            if (nf[d] < 2 * sp.ns) return kErrSpreadBoxSmall;
+        } else {
+            nf[d] = fine_size(ms[d], opts.upsampfac, sp.ns);
+        }
+    }
+    nftot = (long long)nf[0] * nf[1] * nf[2];
+    ntot = (long long)ms[0] * ms[1] * ms[2];
+    if (nftot * maxbatch > 2147483647LL && !opts.gpu_spreadinterponly) {
+        // hipFFT's plan-many interface takes int distances and batch sizes
+        if (nftot > 2147483647LL) return kErrNdataNotValid;
+    }
+
+    // Method: 0 picks SM for type 1 when the tile fits the LDS, GM-sort otherwise.
+    const int bsv[3] = {opts.gpu_binsizex, opts.gpu_binsizey, opts.gpu_binsizez};
+    method = opts.gpu_method;
+    if (method == 0) {
+        method = (type == 1 && sm_tile_bytes(dim, sp.ns, bsv, sizeof(T)) <= kMaxLdsBytes) ? 2 : 1;
+    }
+    if (method == 2 && sm_tile_bytes(dim, sp.ns, bsv, sizeof(T)) > kMaxLdsBytes) return kErrLds;
+
+    for (int d = 0; d < 3; ++d) {
+        geom.nf[d] = nf[d];
+        geom.bs[d] = d < dim ? bsv[d] : 1;
+        if (d >= dim) {
+            geom.nb[d] = 1;
+        } else if (sizeof(T) == 4) {
+            geom.nb[d] = (int)std::ceil((float)nf[d] / (float)geom.bs[d]);
+        } else {
+            geom.nb[d] = (int)std::ceil((double)nf[d] / (double)geom.bs[d]);
+        }
+    }
+    nbins = (long long)geom.nb[0] * geom.nb[1] * geom.nb[2];
+
+    DeviceGuard dg(opts.gpu_device_id);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return kErrHip;
+    if (hipDeviceGetAttribute(&L.ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return kErrHip;
+
+    // Fourier-series factors (FP64 on the host, stored in T on the device).
+    if (!opts.gpu_spreadinterponly) {
+        for (int d = 0; d < dim; ++d) {
+            fseries<T>(nf[d], sp, fser[d]);
+            std::vector<T> h(fser[d].begin(), fser[d].end());
+            if (dmalloc(&fwker[d], h.size())) return kErrHip;
+            if (hipMemcpy(fwker[d], h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) !=
+                hipSuccess)
+                return kErrHip;
+        }
+        if (dmalloc(&fw, (size_t)(nftot * maxbatch))) return kErrHip;
+        int e = make_fft(&fft_full, maxbatch);
+        if (e) return e;
+        fft_full_ok = true;
+    }
+    if (dmalloc(&sb.binsize, nbins) || dmalloc(&sb.binstart, nbins) ||
+        dmalloc(&sb.substart, nbins + 1) || dmalloc(&sb.counters, 2))
+        return kErrHip;
+    const long long nparts = (nbins + kScanChunk - 1) / kScanChunk;
+    if (hipMalloc(&sb.scan_part, sizeof(int2) * (size_t)nparts) != hipSuccess) return kErrHip;
+    return ier;
+}
+
+template <typename T>
+int Plan<T>::setpts(int M_, const T *x, const T *y, const T *z)
+{
+    if (M_ < 0) return kErrNdataNotValid;
+    if (M_ > 0 && (!x || (dim > 1 && !y) || (dim > 2 && !z))) return 1;
+    DeviceGuard dg(opts.gpu_device_id);
+    M = M_;
+    kx = x;
+    ky = dim > 1 ? y : nullptr;
+    kz = dim > 2 ? z : nullptr;
+    if (M > cap_M) {
+        dfree(sb.rank);
+        dfree(sb.binof);
+        dfree(sb.idx);
+        if (dmalloc(&sb.rank, M) || dmalloc(&sb.binof, M) || dmalloc(&sb.idx, M)) {
+            cap_M = -1;
+            return kErrHip;
+        }
+        cap_M = M;
+    }
+    const int maxsub = opts.gpu_maxsubprobsize;
+    sub_bound = nbins + (M + (long long)maxsub - 1) / maxsub;
+    const bool need_sub = method == 2;
+    if (need_sub && sub_bound > cap_sub) {
+        dfree(sb.s2b);
+        if (dmalloc(&sb.s2b, (size_t)sub_bound)) {
+            cap_sub = -1;
+            return kErrHip;
+        }
+        cap_sub = sub_bound;
+    }
+    int e;
+    if (method == 1 && !opts.gpu_sort) {
+        e = run_iota(M, sb.idx, L);
+    } else {
+        e = run_bin_sort<T>(dim, geom, nbins, M, kx, ky, kz, maxsub, need_sub, sb, L);
+    }
+    if (e) return e;
+    pts_set = true;
+    return 0;
+}
+
+template <typename T>
+int Plan<T>::execute(cpx<T> *c, cpx<T> *fk)
+{
+    if (!pts_set) return 1;
+    if ((M > 0 && !c) || !fk) return 1;
+    DeviceGuard dg(opts.gpu_device_id);
+    KerParams<T> kp{sp.beta, sp.c, sp.hw};
+    const int maxsub = opts.gpu_maxsubprobsize;
+    const T *kers[3] = {fwker[0], fwker[1], fwker[2]};
+    int e = 0;
+    if (opts.gpu_spreadinterponly) {
+        // The fine grid is the caller's fk (nf = nmodes): spread into it or
+        // interpolate from it, one batch at a time.
+        for (int b0 = 0; b0 < ntransf; b0 += maxbatch) {
+            const int blk = std::min(maxbatch, ntransf - b0);
+            cpx<T> *cb = c + (long long)b0 * M;
+            cpx<T> *fb = fk + (long long)b0 * nftot;
+            if (type == 1) {
+                if ((e = run_zero<T>(fb, nftot * blk, L))) return e;
+                if ((e = run_spread<T>(method, dim, sp.ns, M, kx, ky, kz, cb, fb, geom, kp, sb,
+                                       maxsub, sub_bound, blk, nftot, L)))
+                    return e;
+            } else {
+                if ((e = run_interp<T>(method, dim, sp.ns, M, kx, ky, kz, cb, fb, geom, kp, sb,
+                                       maxsub, sub_bound, blk, nftot, L)))
+                    return e;
+            }
+        }
+        return 0;
+    }
+    for (int b0 = 0; b0 < ntransf; b0 += maxbatch) {
+        const int blk = std::min(maxbatch, ntransf - b0);
+        cpx<T> *cb = c + (long long)b0 * M;
+        cpx<T> *fb = fk + (long long)b0 * ntot;
+        if (type == 1) {
+            if ((e = run_zero<T>(fw, nftot * blk, L))) return e;
+            if ((e = run_spread<T>(method, dim, sp.ns, M, kx, ky, kz, cb, fw, geom, kp, sb, maxsub,
+                                   sub_bound, blk, nftot, L)))
+                return e;
+            if ((e = fft(fw, blk))) return e;
+            if ((e = run_deconvolve<T>(dim, ms, nf, fw, fb, kers, blk, L))) return e;
+        } else {
+            if ((e = run_amplify<T>(dim, ms, nf, fw, fb, kers, blk, L))) return e;
+            if ((e = fft(fw, blk))) return e;
+            if ((e = run_interp<T>(method, dim, sp.ns, M, kx, ky, kz, cb, fw, geom, kp, sb, maxsub,
+                                   sub_bound, blk, nftot, L)))
+                return e;
+        }
+    }
+    return 0;
+}
+
+template struct Plan<float>;
+template struct Plan<double>;
+
+}  // namespace cfa
+
+struct cufinufft_plan_s : cfa::Plan<double> {};
+struct cufinufftf_plan_s : cfa::Plan<float> {};
+
+namespace {
+
+template <typename P, typename T>
+int makeplan_impl(int type, int dim, int *nmodes, int iflag, int ntransf, T tol, int maxbatch,
+                  P **out, cufinufft_opts *opts)
+{
+    if (!out) return 1;
+    *out = nullptr;
+    P *p = new (std::nothrow) P();
+    if (!p) return 1;
+    const int ier = p->makeplan(type, dim, nmodes, iflag, ntransf, tol, maxbatch, opts);
+    if (ier > 1) {
+        delete p;
+        return ier;
+    }
+    *out = p;
+    return ier;
+}
+
+template <typename P>
+int get_info_impl(P *p, int *out)
+{
+    if (!p || !out) return 1;
+    int total = 0;
+    if (p->pts_set && p->sb.counters) {
+        cfa::DeviceGuard dg(p->opts.gpu_device_id);
+        if (hipStreamSynchronize(p->L.stream) != hipSuccess) return cfa::kErrHip;
+        if (hipMemcpy(&total, p->sb.counters + 1, sizeof(int), hipMemcpyDeviceToHost) !=
+            hipSuccess)
+            return cfa::kErrHip;
+    }
+    const int v[16] = {p->sp.ns,        p->nf[0],         p->nf[1],         p->nf[2],
+                       p->geom.nb[0],   p->geom.nb[1],    p->geom.nb[2],    p->method,
+                       p->geom.bs[0],   p->geom.bs[1],    p->geom.bs[2],    p->opts.gpu_maxsubprobsize,
+                       p->M,            total,            p->maxbatch,      p->opts.gpu_spreadinterponly};
+    std::memcpy(out, v, sizeof(v));
+    return 0;
+}
+
+template <typename P>
+int get_sort_impl(P *p, int *binsize, int *binstart, int *idx, int *substart, int *s2b)
+{
+    if (!p || !p->pts_set) return 1;
+    cfa::DeviceGuard dg(p->opts.gpu_device_id);
+    if (hipStreamSynchronize(p->L.stream) != hipSuccess) return cfa::kErrHip;
+    const size_t nb = (size_t)p->nbins;
+    auto cp = [](int *dst, const int *src, size_t n) {
+        return !dst || !src || n == 0 ||
+               hipMemcpy(dst, src, n * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess;
+    };
+    int total = 0;
+    if (hipMemcpy(&total, p->sb.counters + 1, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return cfa::kErrHip;
+    bool ok = cp(binsize, p->sb.binsize, nb) && cp(binstart, p->sb.binstart, nb) &&
+              cp(idx, p->sb.idx, (size_t)p->M) && cp(substart, p->sb.substart, nb + 1) &&
+              (p->method != 2 || cp(s2b, p->sb.s2b, (size_t)total));
+    return ok ? 0 : cfa::kErrHip;
+}
+
+template <typename P>
+int get_fseries_impl(P *p, double *k1, double *k2, double *k3)
+{
+    if (!p) return 1;
+    double *o[3] = {k1, k2, k3};
+    for (int d = 0; d < p->dim; ++d)
+        if (o[d] && !p->fser[d].empty())
+            std::memcpy(o[d], p->fser[d].data(), p->fser[d].size() * sizeof(double));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cufinufft_default_opts(int type, int dim, cufinufft_opts *opts)
+{
+    return cfa::default_opts(type, dim, opts);
+}
+int cufinufftf_default_opts(int type, int dim, cufinufft_opts *opts)
+{
+    return cfa::default_opts(type, dim, opts);
+}
+
+int cufinufft_makeplan(int type, int dim, int *nmodes, int iflag, int ntransf, double tol,
+                       int maxbatchsize, cufinufft_plan *plan, cufinufft_opts *opts)
+{
+    return makeplan_impl<cufinufft_plan_s, double>(type, dim, nmodes, iflag, ntransf, tol,
+                                                   maxbatchsize, plan, opts);
+}
+int cufinufftf_makeplan(int type, int dim, int *nmodes, int iflag, int ntransf, float tol,
+                        int maxbatchsize, cufinufftf_plan *plan, cufinufft_opts *opts)
+{
+    return makeplan_impl<cufinufftf_plan_s, float>(type, dim, nmodes, iflag, ntransf, tol,
+                                                   maxbatchsize, plan, opts);
+}
+
+int cufinufft_setpts(int M, double *x, double *y, double *z, int, double *, double *, double *,
+                     cufinufft_plan plan)
+{
+    return plan ? plan->setpts(M, x, y, z) : 1;
+}
+int cufinufftf_setpts(int M, float *x, float *y, float *z, int, float *, float *, float *,
+                      cufinufftf_plan plan)
+{
+    return plan ? plan->setpts(M, x, y, z) : 1;
+}
+
+int cufinufft_execute(void *c, void *fk, cufinufft_plan plan)
+{
+    return plan ? plan->execute((cfa::cpx<double> *)c, (cfa::cpx<double> *)fk) : 1;
+}
+int cufinufftf_execute(void *c, void *fk, cufinufftf_plan plan)
+{
+    return plan ? plan->execute((cfa::cpx<float> *)c, (cfa::cpx<float> *)fk) : 1;
+}
+
+int cufinufft_destroy(cufinufft_plan plan)
+{
+    if (!plan) return 1;
+    delete plan;
+    return 0;
+}
+int cufinufftf_destroy(cufinufftf_plan plan)
+{
+    if (!plan) return 1;
+    delete plan;
+    return 0;
+}
+
+int cufinufft_setstream(cufinufft_plan plan, void *stream)
+{
+    if (!plan) return 1;
+    plan->L.stream = (hipStream_t)stream;
+    return 0;
+}
+int cufinufftf_setstream(cufinufftf_plan plan, void *stream)
+{
+    if (!plan) return 1;
+    plan->L.stream = (hipStream_t)stream;
+    return 0;
+}
+
+int cufinufft_get_info(cufinufft_plan plan, int *out) { return get_info_impl(plan, out); }
+int cufinufftf_get_info(cufinufftf_plan plan, int *out) { return get_info_impl(plan, out); }
+
+int cufinufft_get_sort(cufinufft_plan plan, int *binsize, int *binstart, int *idxnupts,
+                       int *subprobstart, int *subprob_to_bin)
+{
+    return get_sort_impl(plan, binsize, binstart, idxnupts, subprobstart, subprob_to_bin);
+}
+int cufinufftf_get_sort(cufinufftf_plan plan, int *binsize, int *binstart, int *idxnupts,
+                        int *subprobstart, int *subprob_to_bin)
+{
+    return get_sort_impl(plan, binsize, binstart, idxnupts, subprobstart, subprob_to_bin);
+}
+
+int cufinufft_get_fseries(cufinufft_plan plan, double *k1, double *k2, double *k3)
+{
+    return get_fseries_impl(plan, k1, k2, k3);
+}
+int cufinufftf_get_fseries(cufinufftf_plan plan, double *k1, double *k2, double *k3)
+{
+    return get_fseries_impl(plan, k1, k2, k3);
+}
+
+}  // extern "C"

@@ -1,0 +1,169 @@
+// Launchers of the bin sort, the dimension dispatch of spread/interp, and the
+// deconvolve / amplify / zero-fill passes.
+#include "nufft_device.cuh"
+#include "nufft_launch.h"
+
+namespace cfa {
+
+namespace {
+inline int ok_or_err() { return hipGetLastError() == hipSuccess ? 0 : kErrHip; }
+
+inline int grid_1d(long long n, int ncu, int per_cu = 32)
+{
+    long long b = (n + 255) / 256;
+    long long cap = (long long)ncu * per_cu;
+    return (int)(b < 1 ? 1 : (b > cap ? cap : b));
+}
+}  // namespace
+
+size_t sm_tile_bytes(int dim, int ns, const int *bs, size_t elem)
+{
+    const int P = (ns + 1) / 2;
+    size_t cells = (size_t)(bs[0] + 2 * P);
+    if (dim > 1) cells *= (size_t)(bs[1] + 2 * P);
+    if (dim > 2) cells *= (size_t)(bs[2] + 2 * P);
+    return 2 * cells * elem;
+}
+
+template <typename T>
+int run_bin_sort(int dim, const BinGeom &g, long long nbins, int M, const T *x, const T *y,
+                 const T *z, int maxsub, bool need_subprob, SortBufs &sb, const Launch &L)
+{
+    k_fill_int<<<grid_1d(nbins, L.ncu), 256, 0, L.stream>>>(nbins, 0, sb.binsize);
+    if (M > 0) {
+        const int gp = grid_1d(M, L.ncu);
+        if (dim == 1)
+            k_bin_count<T, 1><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.binsize, sb.rank,
+                                                        sb.binof);
+        else if (dim == 2)
+            k_bin_count<T, 2><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.binsize, sb.rank,
+                                                        sb.binof);
+        else
+            k_bin_count<T, 3><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.binsize, sb.rank,
+                                                        sb.binof);
+    }
+    const long long nparts = (nbins + kScanChunk - 1) / kScanChunk;
+    int2 *part = static_cast<int2 *>(sb.scan_part);
+    k_scan_reduce<<<(unsigned)nparts, kScanThreads, 0, L.stream>>>(sb.binsize, nbins, maxsub,
+                                                                   part);
+    k_scan_parts<<<1, kScanThreads, 0, L.stream>>>(part, (int)nparts, sb.counters,
+                                                   sb.substart + nbins);
+    k_scan_apply<<<(unsigned)nparts, kScanThreads, 0, L.stream>>>(sb.binsize, nbins, maxsub,
+                                                                  part, sb.binstart, sb.substart);
+    if (M > 0)
+        k_bin_scatter<<<grid_1d(M, L.ncu), 256, 0, L.stream>>>(M, sb.binof, sb.rank,
+                                                               sb.binstart, sb.idx);
+    if (need_subprob)
+        k_map_subprob<<<grid_1d(nbins, L.ncu), 256, 0, L.stream>>>(nbins, sb.substart, sb.s2b);
+    return ok_or_err();
+}
+
+int run_iota(int M, int *idx, const Launch &L)
+{
+    if (M > 0) k_iota<<<grid_1d(M, L.ncu), 256, 0, L.stream>>>(M, idx);
+    return ok_or_err();
+}
+
+template <typename T>
+int run_spread(int method, int dim, int ns, int M, const T *x, const T *y, const T *z,
+               const cpx<T> *c, cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp,
+               const SortBufs &sb, int maxsub, long long sub_bound, int ntr, long long nftot,
+               const Launch &L)
+{
+    if (M == 0) return 0;
+    if (dim == 1)
+        return spread_dim<T, 1>(method, ns, M, x, y, z, c, fw, g, kp, sb, maxsub, sub_bound,
+                                ntr, nftot, L);
+    if (dim == 2)
+        return spread_dim<T, 2>(method, ns, M, x, y, z, c, fw, g, kp, sb, maxsub, sub_bound,
+                                ntr, nftot, L);
+    return spread_dim<T, 3>(method, ns, M, x, y, z, c, fw, g, kp, sb, maxsub, sub_bound, ntr,
+                            nftot, L);
+}
+
+template <typename T>
+int run_interp(int method, int dim, int ns, int M, const T *x, const T *y, const T *z,
+               cpx<T> *c, const cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp,
+               const SortBufs &sb, int maxsub, long long sub_bound, int ntr, long long nftot,
+               const Launch &L)
+{
+    if (M == 0) return 0;
+    if (dim == 1)
+        return interp_dim<T, 1>(method, ns, M, x, y, z, c, fw, g, kp, sb, maxsub, sub_bound,
+                                ntr, nftot, L);
+    if (dim == 2)
+        return interp_dim<T, 2>(method, ns, M, x, y, z, c, fw, g, kp, sb, maxsub, sub_bound,
+                                ntr, nftot, L);
+    return interp_dim<T, 3>(method, ns, M, x, y, z, c, fw, g, kp, sb, maxsub, sub_bound, ntr,
+                            nftot, L);
+}
+
+template <typename T>
+int run_deconvolve(int dim, const int *ms, const int *nf, const cpx<T> *fw, cpx<T> *fk,
+                   const T *const *ker, int ntr, const Launch &L)
+{
+    const long long rows = (long long)ms[1] * ms[2] * ntr;
+    const long long nftot = (long long)nf[0] * nf[1] * nf[2];
+    const int grid = (int)(rows < (long long)L.ncu * 64 ? rows : (long long)L.ncu * 64);
+    if (dim == 1)
+        k_deconvolve<T, 1><<<grid, 256, 0, L.stream>>>(ms[0], ms[1], ms[2], nf[0], nf[1], nf[2],
+                                                       fw, fk, ker[0], ker[1], ker[2], rows,
+                                                       nftot);
+    else if (dim == 2)
+        k_deconvolve<T, 2><<<grid, 256, 0, L.stream>>>(ms[0], ms[1], ms[2], nf[0], nf[1], nf[2],
+                                                       fw, fk, ker[0], ker[1], ker[2], rows,
+                                                       nftot);
+    else
+        k_deconvolve<T, 3><<<grid, 256, 0, L.stream>>>(ms[0], ms[1], ms[2], nf[0], nf[1], nf[2],
+                                                       fw, fk, ker[0], ker[1], ker[2], rows,
+                                                       nftot);
+    return ok_or_err();
+}
+
+template <typename T>
+int run_amplify(int dim, const int *ms, const int *nf, cpx<T> *fw, const cpx<T> *fk,
+                const T *const *ker, int ntr, const Launch &L)
+{
+    const long long rows = (long long)nf[1] * nf[2] * ntr;
+    const long long ntot = (long long)ms[0] * ms[1] * ms[2];
+    const int grid = (int)(rows < (long long)L.ncu * 64 ? rows : (long long)L.ncu * 64);
+    if (dim == 1)
+        k_amplify<T, 1><<<grid, 256, 0, L.stream>>>(ms[0], ms[1], ms[2], nf[0], nf[1], nf[2], fw,
+                                                    fk, ker[0], ker[1], ker[2], rows, ntot);
+    else if (dim == 2)
+        k_amplify<T, 2><<<grid, 256, 0, L.stream>>>(ms[0], ms[1], ms[2], nf[0], nf[1], nf[2], fw,
+                                                    fk, ker[0], ker[1], ker[2], rows, ntot);
+    else
+        k_amplify<T, 3><<<grid, 256, 0, L.stream>>>(ms[0], ms[1], ms[2], nf[0], nf[1], nf[2], fw,
+                                                    fk, ker[0], ker[1], ker[2], rows, ntot);
+    return ok_or_err();
+}
+
+template <typename T>
+int run_zero(cpx<T> *p, long long n, const Launch &L)
+{
+    if (n > 0) k_zero<T><<<grid_1d(n, L.ncu, 64), 256, 0, L.stream>>>(p, n);
+    return ok_or_err();
+}
+
+#define CFA_INST(T)                                                                          \
+    template int run_bin_sort<T>(int, const BinGeom &, long long, int, const T *, const T *, \
+                                 const T *, int, bool, SortBufs &, const Launch &);          \
+    template int run_spread<T>(int, int, int, int, const T *, const T *, const T *,          \
+                               const cpx<T> *, cpx<T> *, const BinGeom &,                    \
+                               const KerParams<T> &, const SortBufs &, int, long long, int,  \
+                               long long, const Launch &);                                   \
+    template int run_interp<T>(int, int, int, int, const T *, const T *, const T *, cpx<T> *, \
+                               const cpx<T> *, const BinGeom &, const KerParams<T> &,        \
+                               const SortBufs &, int, long long, int, long long,             \
+                               const Launch &);                                              \
+    template int run_deconvolve<T>(int, const int *, const int *, const cpx<T> *, cpx<T> *,  \
+                                   const T *const *, int, const Launch &);                   \
+    template int run_amplify<T>(int, const int *, const int *, cpx<T> *, const cpx<T> *,     \
+                                const T *const *, int, const Launch &);                      \
+    template int run_zero<T>(cpx<T> *, long long, const Launch &);
+
+CFA_INST(float)
+CFA_INST(double)
+
+}  // namespace cfa

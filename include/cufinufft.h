@@ -1,0 +1,102 @@
+/* C ABI of the MI355X NUFFT library (libcufinufft.so).
+ *
+ * A drop-in for the type-1 / type-2 entry points of cuFINUFFT v1.3: same
+ * symbol names, argument order, meaning and integer return codes.  Every
+ * pointer to coordinates, strengths or modes is a caller-owned DEVICE pointer
+ * (HIP memory on the plan's device); complex arrays are interleaved {re, im}
+ * (std::complex / hipFloatComplex / hipDoubleComplex layout).
+ *
+ * Return codes (contrib/utils.h:28-36 of the reference, plus this library's
+ * device-side codes, which the reference turned into exit() instead):
+ *   0 ok, 1 generic error or WARN_EPS_TOO_SMALL (makeplan only), 2 bad method,
+ *   3 ERR_SPREAD_BOX_SMALL, 4 ERR_SPREAD_PTS_OUT_RANGE, 5 ERR_SPREAD_ALLOC,
+ *   6 ERR_SPREAD_DIR, 7 ERR_UPSAMPFAC_TOO_SMALL, 8 HORNER_WRONG_BETA,
+ *   9 ERR_NDATA_NOTVALID, 10 HIP runtime error, 11 FFT library error,
+ *   12 tile exceeds the 160 KiB LDS of one workgroup.
+ */
+#ifndef CUFINUFFT_H
+#define CUFINUFFT_H
+
+#include "cufinufft_opts.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Opaque plans (reference: include/cufinufft_eitherprec.h:299-302). */
+typedef struct cufinufft_plan_s *cufinufft_plan;   /* double precision */
+typedef struct cufinufftf_plan_s *cufinufftf_plan; /* single precision */
+
+/* Defaults for a (type, dim): upsampfac 2, method 2 (type 1) or 1 (type 2),
+ * sort on, maxsubprobsize 1024, bins auto, exp(sqrt()) kernel, device 0.
+ * Replaces src/cufinufft.cu:639-730 (cufinufft_default_opts; the single
+ * precision name maps to the same function in the reference macro layer,
+ * include/cufinufft_eitherprec.h:103). */
+int cufinufft_default_opts(int type, int dim, cufinufft_opts *opts);
+int cufinufftf_default_opts(int type, int dim, cufinufft_opts *opts);
+
+/* Plan a transform: include/cufinufft_eitherprec.h:314-316,
+ * src/cufinufft.cu:78-273.  nmodes has 3 entries (unused ones ignored),
+ * maxbatchsize 0 picks min(ntransf, 8), opts may be NULL. */
+int cufinufft_makeplan(int type, int dim, int *nmodes, int iflag, int ntransf,
+                       double tol, int maxbatchsize, cufinufft_plan *plan,
+                       cufinufft_opts *opts);
+int cufinufftf_makeplan(int type, int dim, int *nmodes, int iflag, int ntransf,
+                        float tol, int maxbatchsize, cufinufftf_plan *plan,
+                        cufinufft_opts *opts);
+
+/* Set nonuniform points (device arrays, kept by pointer, not copied, and
+ * re-read by execute) and bin-sort them: include/cufinufft_eitherprec.h:
+ * 317-318, src/cufinufft.cu:275-493.  N, s, t, u are reserved for type 3
+ * and ignored. */
+int cufinufft_setpts(int M, double *x, double *y, double *z, int N, double *s,
+                     double *t, double *u, cufinufft_plan plan);
+int cufinufftf_setpts(int M, float *x, float *y, float *z, int N, float *s,
+                      float *t, float *u, cufinufftf_plan plan);
+
+/* Execute ntransf transforms: c has M*ntransf entries, fk N1*N2(*N3)*ntransf,
+ * transform index slowest, x fastest: include/cufinufft_eitherprec.h:319,
+ * src/cufinufft.cu:495-569.  The pointers are void* here so that C callers
+ * can pass any interleaved complex type. */
+int cufinufft_execute(void *c, void *fk, cufinufft_plan plan);
+int cufinufftf_execute(void *c, void *fk, cufinufftf_plan plan);
+
+/* Free all plan memory and the plan itself: include/cufinufft_eitherprec.h:
+ * 320, src/cufinufft.cu:571-637.  NULL returns 1, as in the reference. */
+int cufinufft_destroy(cufinufft_plan plan);
+int cufinufftf_destroy(cufinufftf_plan plan);
+
+/* ---- Extensions (no counterpart in the reference ABI) ------------------ */
+
+/* Stream for every launch and FFT of this plan (a hipStream_t; NULL is the
+ * null stream, the default).  For callers that order the library with their
+ * own streams, e.g. PyTorch's current stream or a HIP graph capture. */
+int cufinufft_setstream(cufinufft_plan plan, void *stream);
+int cufinufftf_setstream(cufinufftf_plan plan, void *stream);
+
+/* Plan facts, for tests and tuners: out[0..15] = ns, nf1, nf2, nf3,
+ * numbins x, y, z, method, binsize x, y, z, maxsubprobsize, M,
+ * total subproblems of the last setpts (synchronises), maxbatchsize,
+ * spreadinterponly. */
+int cufinufft_get_info(cufinufft_plan plan, int *out);
+int cufinufftf_get_info(cufinufftf_plan plan, int *out);
+
+/* Copy the bin-sort arrays of the last setpts to host memory (synchronous;
+ * any pointer may be NULL): binsize[nbins], binstart[nbins], idxnupts[M],
+ * subprobstart[nbins+1], subprob_to_bin[total subproblems].  The test
+ * surface of the reference's internal arrays (spread*_wrapper.cu). */
+int cufinufft_get_sort(cufinufft_plan plan, int *binsize, int *binstart,
+                       int *idxnupts, int *subprobstart, int *subprob_to_bin);
+int cufinufftf_get_sort(cufinufftf_plan plan, int *binsize, int *binstart,
+                        int *idxnupts, int *subprobstart, int *subprob_to_bin);
+
+/* Fourier-series factors of the plan (host copies of fwkerhalf_d, length
+ * nf_d/2+1, FP64 as computed at plan time; NULL skips an axis). */
+int cufinufft_get_fseries(cufinufft_plan plan, double *k1, double *k2, double *k3);
+int cufinufftf_get_fseries(cufinufftf_plan plan, double *k1, double *k2, double *k3);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,206 @@
+"""GPU parity of the HIP path against the oracle (FP64 CPU restatement) and
+the FP64 direct sum.
+
+Bars:
+* bin sort: per-bin counts, bin starts, subproblem starts and the
+  subproblem->bin map bit-exact; the set of points in each bin exact (order
+  inside a bin is atomic-arrival order, as in the reference, so it is free).
+* spread / interp alone (gpu_spreadinterponly=1): against the oracle's FP64
+  sum, relative l2 <= 2e-6 (FP32) / 1e-12 (FP64).
+* full transforms: relative l2 against the FP64 direct sum on sampled modes
+  / points <= max(2*tol, 1.25 * error of the oracle's own run of the same
+  algorithm); and against the oracle pipeline <= tol/5 (FP32 accumulation
+  rounding is the only difference there).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import gpu_util as G
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+# ---------------------------------------------------------------- bin sort
+SORT_CASES = [
+    # dim, dtype, nmodes (x,y,z), M, distribution, bins
+    (2, np.float32, (256, 256), 100_000, "uniform", None),
+    (2, np.float64, (200, 120), 50_000, "clustered", (16, 8)),
+    (3, np.float32, (64, 64, 64), 200_000, "clustered", None),
+    (3, np.float64, (40, 36, 50), 60_000, "uniform", (8, 8, 4)),
+    (1, np.float32, (1000,), 30_000, "uniform", None),
+]
+
+
+@pytest.mark.parametrize("case", SORT_CASES, ids=lambda c: f"{c[0]}d-{np.dtype(c[1]).name}-{c[4]}")
+def test_bin_sort_exact(torch_dev, case):
+    dim, dtype, nm, M, dist, bins = case
+    _flush(f"bin sort case {case}")
+    pts = (G.uniform_pts if dist == "uniform" else G.clustered_pts)(M, dim, dtype, seed=11)
+    opts = dict(gpu_method=2, gpu_maxsubprobsize=64)
+    if bins:
+        for k, v in zip(("gpu_binsizex", "gpu_binsizey", "gpu_binsizez"), bins):
+            opts[k] = v
+    from cufinufft_amd import cufinufft
+    plan = cufinufft(1, tuple(nm[::-1]), eps=1e-5 if dtype == np.float32 else 1e-9,
+                     dtype=dtype, **opts)
+    plan.set_pts(*[G.to_dev(p, torch_dev) for p in pts[::-1]])
+    got = plan.sort_arrays()
+    inf = plan.info()
+    ref = O.bin_sort(pts, inf["nf"][:dim], inf["binsize"][:dim], maxsub=64)
+    assert tuple(ref["numbins"]) == tuple(inf["numbins"][:dim])
+    np.testing.assert_array_equal(got["binsize"], ref["binsize"])
+    np.testing.assert_array_equal(got["binstart"], ref["binstart"])
+    np.testing.assert_array_equal(got["substart"], ref["substart"])
+    assert inf["nsubprob"] == ref["total"]
+    np.testing.assert_array_equal(got["s2b"], np.repeat(np.arange(ref["binsize"].size), ref["numsub"]))
+    # same set of points in every bin; idx is a permutation
+    idx = got["idx"]
+    assert np.array_equal(np.sort(idx), np.arange(M))
+    np.testing.assert_array_equal(ref["binof"][idx], np.repeat(np.arange(ref["binsize"].size), ref["binsize"]))
+
+
+# ------------------------------------------------------- spread/interp only
+SI_CASES = [
+    (2, np.float32, (64, 48), 5000, 1e-5),
+    (2, np.float64, (40, 52), 3000, 1e-9),
+    (3, np.float32, (24, 20, 28), 4000, 1e-6),
+    (3, np.float64, (24, 24, 24), 3000, 1e-9),
+    (1, np.float64, (100,), 2000, 1e-8),
+]
+
+
+@pytest.mark.parametrize("method", [1, 2])
+@pytest.mark.parametrize("case", SI_CASES, ids=lambda c: f"{c[0]}d-{np.dtype(c[1]).name}")
+def test_spread_only(torch_dev, case, method):
+    dim, dtype, nm, M, tol = case
+    _flush(f"spread-only {case} method {method}")
+    pts = G.uniform_pts(M, dim, dtype, seed=3)
+    c = G.strengths(M, dtype, seed=3)
+    fw, plan = G.run_type1(torch_dev, pts, c, nm, tol, dtype, gpu_method=method,
+                           gpu_spreadinterponly=1, gpu_binsizex=8,
+                           gpu_binsizey=8 if dim > 1 else 1, gpu_binsizez=4 if dim > 2 else 1)
+    sp = O.setup_spreader(tol, dtype == np.float32)
+    ref = O.spread(pts, c, list(nm), sp)
+    bar = 2e-6 if dtype == np.float32 else 1e-12
+    assert rel_l2(fw, ref) < bar
+
+
+@pytest.mark.parametrize("method", [1, 2])
+@pytest.mark.parametrize("case", SI_CASES, ids=lambda c: f"{c[0]}d-{np.dtype(c[1]).name}")
+def test_interp_only(torch_dev, case, method):
+    dim, dtype, nm, M, tol = case
+    _flush(f"interp-only {case} method {method}")
+    pts = G.uniform_pts(M, dim, dtype, seed=4)
+    fw = G.modes(tuple(nm[::-1]), dtype, seed=4)
+    c, plan = G.run_type2(torch_dev, pts, fw, tol, dtype, gpu_method=method,
+                          gpu_spreadinterponly=1, gpu_binsizex=8,
+                          gpu_binsizey=8 if dim > 1 else 1, gpu_binsizez=4 if dim > 2 else 1)
+    sp = O.setup_spreader(tol, dtype == np.float32)
+    ref = O.interp(pts, fw, list(nm), sp)
+    bar = 2e-6 if dtype == np.float32 else 1e-12
+    assert rel_l2(c, ref) < bar
+
+
+# ----------------------------------------------------------- full NUFFTs
+NU_CASES = [
+    # dim, dtype, nmodes (x,y,z), M, tol, distribution
+    (2, np.float32, (256, 256), 100_000, 1e-5, "uniform"),   # the 2D type-1 workload shape
+    (2, np.float32, (64, 80), 20_000, 1e-3, "clustered"),
+    (2, np.float64, (70, 64), 20_000, 1e-9, "uniform"),
+    (3, np.float32, (32, 32, 32), 50_000, 1e-5, "clustered"),
+    (3, np.float64, (32, 24, 28), 30_000, 1e-9, "clustered"),
+    (3, np.float64, (16, 16, 16), 4096, 1e-3, "uniform"),
+    (1, np.float64, (500,), 5000, 1e-10, "uniform"),
+]
+
+
+def _pts(case, seed):
+    dim, dtype, nm, M, tol, dist = case
+    return (G.uniform_pts if dist == "uniform" else G.clustered_pts)(M, dim, dtype, seed)
+
+
+def _bar(tol, oracle_err):
+    return max(2.0 * tol, 1.25 * oracle_err)
+
+
+@pytest.mark.parametrize("method", [1, 2])
+@pytest.mark.parametrize("case", NU_CASES, ids=lambda c: f"{c[0]}d-{np.dtype(c[1]).name}-{c[4]:g}")
+def test_type1(torch_dev, case, method):
+    dim, dtype, nm, M, tol, dist = case
+    _flush(f"type1 {case} method {method}")
+    pts = _pts(case, seed=21)
+    c = G.strengths(M, dtype, seed=21)
+    fk, plan = G.run_type1(torch_dev, pts, c, nm, tol, dtype, gpu_method=method)
+    rng = np.random.default_rng(5)
+    ntot = int(np.prod(nm))
+    kidx = rng.choice(ntot, min(1000, ntot), replace=False)
+    direct = O.dirft_type1([p.astype(np.float64) for p in pts], c, list(nm), +1, kidx)
+    P = O.Plan(1, list(nm), tol, single=dtype == np.float32)
+    ofk = P.type1(pts, c)
+    oerr = rel_l2(ofk.ravel()[kidx], direct)
+    err = rel_l2(fk.ravel()[kidx], direct)
+    _flush(f"   gpu err {err:.3e} oracle err {oerr:.3e}")
+    assert err <= _bar(tol, oerr)
+    assert rel_l2(fk, ofk) <= tol / 5
+
+
+@pytest.mark.parametrize("method", [1, 2])
+@pytest.mark.parametrize("case", NU_CASES, ids=lambda c: f"{c[0]}d-{np.dtype(c[1]).name}-{c[4]:g}")
+def test_type2(torch_dev, case, method):
+    dim, dtype, nm, M, tol, dist = case
+    _flush(f"type2 {case} method {method}")
+    pts = _pts(case, seed=22)
+    fk = G.modes(tuple(nm[::-1]), dtype, seed=22)
+    c, plan = G.run_type2(torch_dev, pts, fk, tol, dtype, gpu_method=method)
+    rng = np.random.default_rng(6)
+    pidx = rng.choice(M, min(1000, M), replace=False)
+    direct = O.dirft_type2([p.astype(np.float64) for p in pts], fk, list(nm), -1, pidx)
+    P = O.Plan(2, list(nm), tol, single=dtype == np.float32)
+    oc = P.type2(pts, fk)
+    oerr = rel_l2(oc[pidx], direct)
+    err = rel_l2(c[pidx], direct)
+    _flush(f"   gpu err {err:.3e} oracle err {oerr:.3e}")
+    assert err <= _bar(tol, oerr)
+    assert rel_l2(c, oc) <= tol / 5
+
+
+def test_many_batched(torch_dev):
+    """ntransf=5 through maxbatchsize=2 (a partial last batch), type 1 and 2."""
+    dtype, nm, M, tol, ntr = np.float32, (48, 40), 8000, 1e-5, 5
+    _flush("many batched")
+    pts = G.uniform_pts(M, 2, dtype, seed=7)
+    c = G.strengths(M, dtype, seed=7, ntr=ntr)
+    fk, _ = G.run_type1(torch_dev, pts, c, nm, tol, dtype, ntr=ntr, maxbatchsize=2)
+    P = O.Plan(1, list(nm), tol, single=True)
+    for t in range(ntr):
+        ref = P.type1(pts, c[t * M:(t + 1) * M])
+        assert rel_l2(fk[t], ref) <= tol / 5
+    fin = G.modes(tuple(nm[::-1]), dtype, seed=7, ntr=ntr)
+    cc, _ = G.run_type2(torch_dev, pts, fin, tol, dtype, ntr=ntr, maxbatchsize=2)
+    P2 = O.Plan(2, list(nm), tol, single=True)
+    for t in range(ntr):
+        ref = P2.type2(pts, fin[t])
+        assert rel_l2(cc[t * M:(t + 1) * M], ref) <= tol / 5
+
+
+def test_setpts_twice(torch_dev):
+    """Re-setpts on one plan (test/cufinufft2d1nupts_test.cu:145-186)."""
+    import torch
+    from cufinufft_amd import cufinufft
+    dtype, nm, tol = np.float64, (32, 30), 1e-8
+    _flush("setpts twice")
+    plan = cufinufft(1, nm[::-1], eps=tol, dtype=dtype)
+    P = O.Plan(1, list(nm), tol, single=False)
+    for seed, M in ((1, 3000), (2, 7000), (3, 1000)):
+        pts = G.uniform_pts(M, 2, dtype, seed)
+        c = G.strengths(M, dtype, seed)
+        plan.set_pts(*[G.to_dev(p, torch_dev) for p in pts[::-1]])
+        fk = torch.zeros(nm[::-1], dtype=torch.complex128, device=torch_dev)
+        plan.execute(G.to_dev(c, torch_dev), fk)
+        assert rel_l2(fk.cpu().numpy(), P.type1(pts, c)) < 1e-9
