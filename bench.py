@@ -1,0 +1,401 @@
+"""Benchmark of the NUFFT execute path on one MI355X.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3|c3d|c4]
+                [--no-cpu] [--no-tune]
+
+A step is one ``execute`` of the plan (one pass of the hot path over one batch
+of synthetic input, with points, strengths and modes already resident in
+HBM).  The default workload is the 3D type-2 double-precision transform at
+tol 1e-9 with N = 128^3 modes and M = 1e7 points drawn from 8 wrapped Gaussian
+clusters (sigma 0.3), seed 2.  Prints one JSON line:
+
+* ``value``: NU points per second per execute (M * ntransf / t_execute) of
+  the tuned plan (the autotuner's choice of method / bins / subproblem size);
+  ``plain`` holds the same number for the reference's default options.
+* ``rel_l2``: error of both against an FP64 direct sum at 1000 sampled
+  points or modes (exact, computed separably on the CPU).
+* ``roofline``: the dominant kernel (spread or interp) of the tuned plan,
+  algorithmic bytes per launch / its mean launch time from HIP events on the
+  plan's stream over the timed region, against the 8 TB/s HBM peak.
+* ``cpu_baseline``: the same pipeline on the host cores (the CPU restatement in
+  oracle/, FP64, OpenMP interp/spread + scipy FFT), on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "c2": dict(type=2, dim=3, N=(128, 128, 128), M=10_000_000, tol=1e-9, dtype="float64",
+               dist="clusters8", ntransf=1, seed=2,
+               desc="3D type-2 FP64 tol 1e-9, N=128^3, M=1e7 in 8 Gaussian clusters (sigma 0.3)"),
+    "c1": dict(type=1, dim=2, N=(256, 256), M=100_000, tol=1e-5, dtype="float32",
+               dist="uniform", ntransf=1, seed=1,
+               desc="2D type-1 FP32 tol 1e-5, N=256^2, M=1e5 uniform"),
+    "c3": dict(type=1, dim=3, N=(256, 256, 256), M=100_000_000, tol=1e-5, dtype="float32",
+               dist="mixed", ntransf=1, seed=3,
+               desc="3D type-1 FP32 tol 1e-5, N=256^3, M=1e8 (50% uniform, 50% in 4 clusters sigma 0.2)"),
+    "c3d": dict(type=1, dim=3, N=(256, 256, 256), M=100_000_000, tol=1e-9, dtype="float64",
+                dist="mixed", ntransf=1, seed=3,
+                desc="3D type-1 FP64 tol 1e-9, N=256^3, M=1e8 (50% uniform, 50% in 4 clusters sigma 0.2)"),
+    "c4": dict(type=1, dim=2, N=(512, 512), M=2_000_000, tol=1e-6, dtype="float32",
+               dist="uniform", ntransf=32, seed=4,
+               desc="2D type-1 FP32 tol 1e-6, N=512^2, M=2e6 uniform, ntransf=32"),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------- inputs
+def make_points(w, rng):
+    M, dim = w["M"], w["dim"]
+    dt = np.dtype(w["dtype"])
+    if w["dist"] == "uniform":
+        P = rng.uniform(-np.pi, np.pi, (dim, M))
+    elif w["dist"] == "clusters8":
+        cen = rng.uniform(-np.pi, np.pi, (8, dim))
+        lab = rng.integers(0, 8, M)
+        P = (cen[lab] + 0.3 * rng.standard_normal((M, dim))).T
+    elif w["dist"] == "mixed":
+        Mu = M // 2
+        cen = rng.uniform(-np.pi, np.pi, (4, dim))
+        lab = rng.integers(0, 4, M - Mu)
+        P = np.concatenate([rng.uniform(-np.pi, np.pi, (dim, Mu)),
+                            (cen[lab] + 0.2 * rng.standard_normal((M - Mu, dim))).T], axis=1)
+    else:
+        raise ValueError(w["dist"])
+    P = np.mod(P + np.pi, 2 * np.pi) - np.pi
+    return [np.ascontiguousarray(P[d]).astype(dt) for d in range(dim)]
+
+
+def cplx_normal(rng, shape, dt):
+    cd = np.complex64 if dt == np.float32 else np.complex128
+    a = np.empty(shape, cd)
+    a.real = rng.standard_normal(shape)
+    a.imag = rng.standard_normal(shape)
+    return a
+
+
+# ------------------------------------------------------- exact references
+def direct_type2(pts, fk, iflag, pidx):
+    """Exact FP64 type-2 sum at points pidx: separable contraction
+    c_j = sum_k3 e3 sum_k2 e2 sum_k1 e1 f[k3,k2,k1] (three GEMMs)."""
+    dim = len(pts)
+    sg = 1.0 if iflag >= 0 else -1.0
+    f = fk.astype(np.complex128)
+    Ns = f.shape[::-1]  # (N1, N2, N3)
+    E = [np.exp(1j * sg * np.outer(pts[d][pidx].astype(np.float64),
+                                   np.arange(-(Ns[d] // 2), (Ns[d] + 1) // 2)))
+         for d in range(dim)]
+    A = f.reshape(-1, Ns[0]) @ E[0].T  # (rest, P)
+    if dim == 1:
+        return A[0]
+    A = A.reshape(-1, Ns[1], len(pidx))
+    B = np.einsum("abp,pb->ap", A, E[1])
+    if dim == 2:
+        return B[0]
+    return np.einsum("ap,pa->p", B, E[2])
+
+
+def direct_type1(pts, c, N, iflag, kidx, chunk=200_000):
+    """Exact FP64 type-1 sum at flat mode indices kidx (x fastest)."""
+    dim = len(pts)
+    sg = 1.0 if iflag >= 0 else -1.0
+    ks = []
+    rem = np.asarray(kidx)
+    for d in range(dim):
+        ks.append(rem % N[d] - N[d] // 2)
+        rem = rem // N[d]
+    out = np.zeros(len(kidx), np.complex128)
+    M = pts[0].size
+    for s in range(0, M, chunk):
+        ph = np.zeros((len(kidx), min(chunk, M - s)))
+        for d in range(dim):
+            ph += np.outer(ks[d], pts[d][s:s + chunk].astype(np.float64))
+        out += np.exp(1j * sg * ph) @ c[s:s + chunk].astype(np.complex128)
+    return out
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def measured_traffic(workload, kernel_prefix, opts):
+    """HBM bytes per launch of the kernel for this workload and options, from
+    the TRAFFIC records of the committed rocprofv3 summaries in profiles/
+    (FETCH_SIZE and WRITE_SIZE runs scaled by a plain-copy calibration), or
+    None when no summary matches."""
+    root = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+    if not os.path.isdir(root):
+        return None
+    for fn in sorted(os.listdir(root)):
+        if not fn.endswith(".md"):
+            continue
+        for line in open(os.path.join(root, fn)):
+            if not line.startswith("TRAFFIC "):
+                continue
+            rec = json.loads(line[len("TRAFFIC "):])
+            if (rec.get("workload") == workload and rec.get("opts") == opts
+                    and rec.get("kernel", "").startswith(kernel_prefix)):
+                return float(rec["bytes"])
+    return None
+
+
+# ---------------------------------------------------------- GPU timing
+def algorithmic_bytes(w, nf, kind):
+    """Algorithmic HBM bytes per launch of the dominant kernel.
+    spread: T*M*2s (strengths) + M*d*s (coords) + 4M (sorted index)
+            + T*nf_tot*2s (grid written once);
+    interp: T*nf_tot*2s (grid read once) + M*d*s + 4M + T*M*2s (output)."""
+    s = 4 if w["dtype"] == "float32" else 8
+    M, d, T = w["M"], w["dim"], w["ntransf"]
+    nftot = int(np.prod(nf[:d]))
+    return T * M * 2 * s + M * d * s + 4 * M + T * nftot * 2 * s
+
+
+def run_plan(w, dev, dpts, din, opts, steps, warmup, maxbatch):
+    import torch
+    from cufinufft_amd import cufinufft
+    dt = np.dtype(w["dtype"])
+    shape_np = tuple(w["N"][::-1])
+    plan = cufinufft(w["type"], shape_np, n_trans=w["ntransf"], eps=w["tol"], dtype=dt,
+                     maxbatchsize=maxbatch, **opts)
+    # setpts throughput (median of 3)
+    plan.set_pts(*dpts[::-1])
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        plan.set_pts(*dpts[::-1])
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    t_setpts = float(np.median(ts))
+    M, T = w["M"], w["ntransf"]
+    cd = torch.complex64 if dt == np.float32 else torch.complex128
+    if w["type"] == 1:
+        out = torch.empty(((T,) if T > 1 else ()) + shape_np, dtype=cd, device=dev)
+        args = (din, out)
+    else:
+        out = torch.empty(M * T, dtype=cd, device=dev)
+        args = (out, din)
+    for _ in range(warmup):
+        plan.execute(*args)
+    torch.cuda.synchronize()
+    plan.timing()  # clear
+    plan.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.execute(*args)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    plan.set_timing(False)
+    stages = plan.timing()
+    return dict(plan=plan, out=out, t_exec=t, t_setpts=t_setpts, stages=stages,
+                info=plan.info())
+
+
+def check_accuracy(w, pts, host_in, out, rng):
+    T = w["ntransf"]
+    if w["type"] == 2:
+        pidx = rng.choice(w["M"], 1000, replace=False)
+        fk0 = host_in[0] if T > 1 else host_in
+        ref = direct_type2(pts, fk0, -1, pidx)
+        got = out[:w["M"]].cpu().numpy()[pidx]
+    else:
+        ntot = int(np.prod(w["N"]))
+        kidx = rng.choice(ntot, 1000, replace=False)
+        ref = direct_type1(pts, host_in[:w["M"]], w["N"], +1, kidx)
+        o = out.cpu().numpy()
+        got = (o[0] if T > 1 else o).ravel()[kidx]
+    return rel_l2(got, ref)
+
+
+# -------------------------------------------------------- CPU baseline
+def cpu_baseline(w, pts, host_in, budget_pts):
+    """The same pipeline on the host cores (oracle/: FP64, OpenMP
+    interp/spread, scipy FFT with all workers) on a bounded sample of the
+    points; the fixed grid cost (amplify/deconvolve + FFT) is timed at full
+    size and the per-point cost is extrapolated to M."""
+    from oracle import oracle as O
+    single = w["dtype"] == "float32"
+    cores = O.num_threads()
+    P = O.Plan(w["type"], list(w["N"]), w["tol"], single=single)
+    Ms = min(w["M"], budget_pts)
+    sub = [p[:Ms] for p in pts]
+    import scipy.fft as sfft
+    # fixed cost: grid-sized stages at full size
+    t0 = time.perf_counter()
+    if w["type"] == 2:
+        fk0 = host_in[0] if w["ntransf"] > 1 else host_in
+        fw = O.amplify(fk0, P.ms, P.nf, P.kers)
+        fw = O._fft(fw, P.iflag, cores)
+    else:
+        fw = np.zeros(tuple(P.nf[::-1]), np.complex128)
+        fw = O._fft(fw, P.iflag, cores)
+        O.deconvolve(fw, P.ms, P.nf, P.kers)
+    t_fixed = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    if w["type"] == 2:
+        O.interp(sub, fw, P.nf, P.sp)
+    else:
+        O.spread(sub, host_in[:Ms], P.nf, P.sp)
+    t_pts = time.perf_counter() - t0
+    t_full = t_fixed + t_pts * (w["M"] / Ms)
+    T = w["ntransf"]
+    value = w["M"] * T / (t_full * T)
+    threads = cores if w["type"] == 2 else 1
+    return dict(value=value, unit="NU pts/s", cores=threads, kind="port",
+                sample=(f"oracle/ FP64 pipeline: grid stages (amplify/deconvolve + scipy FFT, "
+                        f"{cores} workers) timed at full size ({t_fixed:.2f} s), "
+                        f"{'interp (OpenMP)' if w['type'] == 2 else 'spread (1 thread)'} timed on "
+                        f"{Ms} of {w['M']} points ({t_pts:.2f} s) and extrapolated; "
+                        f"one transform"))
+
+
+# ---------------------------------------------------------------- main
+def candidates(w):
+    """Tuner search space for the workload (method x bins x subproblem size)."""
+    dim = w["dim"]
+    c = []
+    if dim == 3:
+        bins = [(16, 16, 2), (8, 8, 8), (8, 8, 4), (16, 8, 4), (4, 4, 4)]
+    elif dim == 2:
+        bins = [(32, 32, 1), (16, 16, 1), (64, 32, 1), (64, 64, 1)]
+    else:
+        bins = [(1024, 1, 1), (512, 1, 1)]
+    for b in bins:
+        for meth in (1, 2):
+            for msub in ((1024,) if meth == 1 else (1024, 256)):
+                c.append(dict(gpu_method=meth, gpu_binsizex=b[0], gpu_binsizey=b[1],
+                              gpu_binsizez=b[2], gpu_maxsubprobsize=msub))
+    return c
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--opts", default=None,
+                    help="JSON dict of cufinufft_opts fields for the tuned plan (skips the sweep)")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="only run a plain device copy of this many bytes (counter calibration)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(--gpus {a.gpus} requested); run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    dev = torch.device("cuda:0")
+    if a.copy_calib:
+        src = torch.ones(a.copy_calib // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copied {a.copy_calib} bytes x3")
+        return 0
+    w = WORKLOADS[a.workload]
+    dt = np.dtype(w["dtype"])
+    rng = np.random.default_rng(w["seed"])
+    log(f"[bench] workload {a.workload}: {w['desc']}")
+    pts = make_points(w, rng)
+    T = w["ntransf"]
+    if w["type"] == 1:
+        host_in = cplx_normal(rng, (w["M"] * T,), dt)
+    else:
+        host_in = cplx_normal(rng, ((T,) if T > 1 else ()) + tuple(w["N"][::-1]), dt)
+    dpts = [torch.from_numpy(p).to(dev) for p in pts]
+    din = torch.from_numpy(host_in).to(dev)
+    torch.cuda.synchronize()
+
+    maxbatch = min(T, 8)
+    log("[bench] plain plan (reference default options)")
+    plain = run_plan(w, dev, dpts, din, {}, a.steps, a.warmup, maxbatch)
+    err_plain = check_accuracy(w, pts, host_in, plain["out"], np.random.default_rng(99))
+    log(f"[bench] plain: {plain['t_exec'] * 1e3:.3f} ms/exec, rel_l2 {err_plain:.3e}, "
+        f"info {plain['info']}")
+    best, best_opts = plain, {}
+    tried = []
+    if a.opts:
+        best_opts = json.loads(a.opts)
+        log(f"[bench] given options {best_opts}")
+        best = run_plan(w, dev, dpts, din, best_opts, a.steps, a.warmup, maxbatch)
+    elif not a.no_tune:
+        for opts in candidates(w):
+            try:
+                r = run_plan(w, dev, dpts, din, opts, max(2, a.steps // 3), 1, maxbatch)
+            except RuntimeError as e:
+                log(f"[bench]   {opts}: {e}")
+                continue
+            tried.append((opts, r["t_exec"]))
+            log(f"[bench]   {opts}: {r['t_exec'] * 1e3:.3f} ms")
+            if r["t_exec"] < best["t_exec"]:
+                best, best_opts = r, opts
+            del r
+        if best is not plain:
+            log(f"[bench] tuned choice {best_opts}; re-timing")
+            best = run_plan(w, dev, dpts, din, best_opts, a.steps, a.warmup, maxbatch)
+    err_best = check_accuracy(w, pts, host_in, best["out"], np.random.default_rng(99))
+    M = w["M"]
+    value = M * T / best["t_exec"]
+    st = best["stages"]["spread_interp"]
+    kern_ms = st[0] / max(st[1], 1)
+    nbytes = algorithmic_bytes(w, best["info"]["nf"], "interp" if w["type"] == 2 else "spread")
+    achieved = nbytes / (kern_ms * 1e-3) / 1e9
+    kname = ("k_interp_" if w["type"] == 2 else "k_spread_") + \
+        ("gm" if best["info"]["method"] == 1 else "sm")
+    ktmpl = f"void cfa::{kname}<{'float' if dt == np.float32 else 'double'}, {w['dim']}, " \
+            f"{best['info']['ns']}>"
+    traffic = measured_traffic(a.workload, ktmpl, best_opts)
+    res = {
+        "metric": "NU pts/s per execute at rel-l2 <= tol: tuned/batched vs plain, 1 MI355X",
+        "value": value,
+        "unit": "NU pts/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": best["t_exec"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if dt == np.float32 else "f64",
+        "data": "synthetic",
+        "config": {"workload": w["desc"], "type": w["type"], "dim": w["dim"],
+                   "n_modes": list(w["N"]), "M": M, "ntransf": T, "tol": w["tol"],
+                   "tuned_opts": best_opts or "reference defaults"},
+        "rel_l2": {"tuned": err_best, "plain": err_plain, "tol": w["tol"]},
+        "plain": {"value": M * T / plain["t_exec"], "ms_per_step": plain["t_exec"] * 1e3,
+                  "method": plain["info"]["method"]},
+        "setpts": {"pts_per_s": M / best["t_setpts"], "ms": best["t_setpts"] * 1e3},
+        "stages_ms_per_exec": {k: v[0] / max(a.steps, 1) for k, v in best["stages"].items()},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "algorithmic_bytes": nbytes, "kernel_ms": kern_ms},
+        "cpu_baseline": None,
+    }
+    del plain, best
+    torch.cuda.synchronize()
+    if not a.no_cpu:
+        log("[bench] CPU baseline")
+        res["cpu_baseline"] = cpu_baseline(w, pts, host_in, a.cpu_sample)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -24,7 +24,7 @@ LIB_PATH = os.environ.get("CUFINUFFT_AMD_LIB", os.path.join(_HERE, "libcufinufft
 if not os.path.exists(LIB_PATH):
     raise RuntimeError(
         f"libcufinufft.so not found at {LIB_PATH}; build it with "
-        "`python -m cufinufft_amd.build` (hipcc, gfx950)")
+        "`python cufinufft_amd/build.py` (hipcc, gfx950)")
 lib = ctypes.cdll.LoadLibrary(LIB_PATH)
 
 
@@ -138,6 +138,21 @@ _get_fseriesf = lib.cufinufftf_get_fseries
 _get_fseriesf.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
 _get_fseriesf.restype = c_int
 
+_set_timing = lib.cufinufft_set_timing
+_set_timing.argtypes = [c_void_p, c_int]
+_set_timing.restype = c_int
+_set_timingf = lib.cufinufftf_set_timing
+_set_timingf.argtypes = [c_void_p, c_int]
+_set_timingf.restype = c_int
+_get_timing = lib.cufinufft_get_timing
+_get_timing.argtypes = [c_void_p, c_void_p, c_void_p]
+_get_timing.restype = c_int
+_get_timingf = lib.cufinufftf_get_timing
+_get_timingf.argtypes = [c_void_p, c_void_p, c_void_p]
+_get_timingf.restype = c_int
+
+STAGES = ("sort", "grid_zero", "spread_interp", "fft", "deconv_amplify")
+
 EXPORTED = [
     "cufinufft_default_opts", "cufinufftf_default_opts",
     "cufinufft_makeplan", "cufinufftf_makeplan",
@@ -148,4 +163,6 @@ EXPORTED = [
     "cufinufft_get_info", "cufinufftf_get_info",
     "cufinufft_get_sort", "cufinufftf_get_sort",
     "cufinufft_get_fseries", "cufinufftf_get_fseries",
+    "cufinufft_set_timing", "cufinufftf_set_timing",
+    "cufinufft_get_timing", "cufinufftf_get_timing",
 ]

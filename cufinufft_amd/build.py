@@ -1,6 +1,7 @@
 """Build libcufinufft.so in-tree with hipcc for gfx950.
 
-Usage: python -m cufinufft_amd.build [-j N]
+Usage: python cufinufft_amd/build.py [-j N]   (a file script: it never imports
+the package, which needs the library it builds)
 
 Each translation unit under csrc/ is compiled to an object in parallel, then
 linked with hipFFT into cufinufft_amd/libcufinufft.so (the library the ctypes

@@ -30,6 +30,14 @@ constexpr int kErrHornerWrongBeta = 8;
 constexpr int kErrNdataNotValid = 9;
 constexpr int kMaxNspread = 16;
 
+// Stage ids of the event timer (cufinufft_get_timing).
+constexpr int kStageSort = 0;           // setpts: bin sort / index setup
+constexpr int kStageGrid = 1;           // type 1: fine-grid zero fill
+constexpr int kStageSpreadInterp = 2;   // spread (type 1) or interp (type 2)
+constexpr int kStageFft = 3;            // hipFFT
+constexpr int kStageDeconv = 4;         // deconvolve (type 1) or amplify (type 2)
+constexpr int kNumStages = 8;
+
 // ---------------------------------------------------------------------------
 // Spreader parameters, contrib/spreadinterp.cpp:6-67, evaluated in the plan
 // precision T exactly as the reference's overloads resolve (the single
@@ -244,6 +252,7 @@ struct Plan {
     int nf[3] = {1, 1, 1};
     int ntransf = 1, maxbatch = 1, iflag = 1;
     int method = 1;
+    bool eps_warning = false;
     BinGeom geom{};
     long long nbins = 1;
     long long nftot = 1, ntot = 1;
@@ -267,11 +276,53 @@ struct Plan {
     bool fft_full_ok = false, fft_rem_ok = false;
     Launch L;
 
+    // Stage timer: HIP events recorded on the plan stream around every stage
+    // while enabled; read (and cleared) by get_timing after a synchronise.
+    struct TimedSpan {
+        int stage;
+        hipEvent_t a, b;
+    };
+    bool timing = false;
+    std::vector<TimedSpan> spans;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    hipEvent_t ev_open = nullptr;
+    static constexpr size_t kMaxSpans = 1 << 16;
+
+    hipEvent_t next_event()
+    {
+        if (ev_used < ev_pool.size()) return ev_pool[ev_used++];
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ev_pool.push_back(e);
+        ++ev_used;
+        return e;
+    }
+    void tbegin()
+    {
+        if (!timing || spans.size() >= kMaxSpans) return;
+        ev_open = next_event();
+        if (ev_open) hipEventRecord(ev_open, L.stream);
+    }
+    void tend(int stage)
+    {
+        if (!timing || !ev_open) return;
+        hipEvent_t b = next_event();
+        if (!b) return;
+        hipEventRecord(b, L.stream);
+        spans.push_back({stage, ev_open, b});
+        ev_open = nullptr;
+    }
+
     ~Plan() { release(); }
 
     void release()
     {
         DeviceGuard dg(opts.gpu_device_id);
+        for (auto e : ev_pool) hipEventDestroy(e);
+        ev_pool.clear();
+        spans.clear();
+        ev_used = 0;
         if (fft_full_ok) hipfftDestroy(fft_full);
         if (fft_rem_ok) hipfftDestroy(fft_rem);
         fft_full_ok = fft_rem_ok = false;
@@ -388,8 +439,9 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
     if (ntransf_ < 1) return kErrNdataNotValid;
     if (opts.gpu_method < 0 || opts.gpu_method > 2) return 2;
     if (opts.gpu_maxsubprobsize < 1) return 1;
-    int ier = setup_spreader<T>(sp, tol, (T)opts.upsampfac, opts.gpu_kerevalmeth);
+    const int ier = setup_spreader<T>(sp, tol, (T)opts.upsampfac, opts.gpu_kerevalmeth);
     if (ier > 1) return ier;
+    eps_warning = ier == kWarnEpsTooSmall;
     if (opts.gpu_kerevalmeth != 0 && opts.gpu_kerevalmeth != 1) return 1;
 
     default_binsizes(dim, opts);
@@ -462,7 +514,7 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
         return kErrHip;
     const long long nparts = (nbins + kScanChunk - 1) / kScanChunk;
     if (hipMalloc(&sb.scan_part, sizeof(int2) * (size_t)nparts) != hipSuccess) return kErrHip;
-    return ier;
+    return 0;
 }
 
 template <typename T>
@@ -497,11 +549,13 @@ int Plan<T>::setpts(int M_, const T *x, const T *y, const T *z)
         cap_sub = sub_bound;
     }
     int e;
+    tbegin();
     if (method == 1 && !opts.gpu_sort) {
         e = run_iota(M, sb.idx, L);
     } else {
         e = run_bin_sort<T>(dim, geom, nbins, M, kx, ky, kz, maxsub, need_sub, sb, L);
     }
+    tend(kStageSort);
     if (e) return e;
     pts_set = true;
     return 0;
@@ -542,20 +596,59 @@ int Plan<T>::execute(cpx<T> *c, cpx<T> *fk)
         cpx<T> *cb = c + (long long)b0 * M;
         cpx<T> *fb = fk + (long long)b0 * ntot;
         if (type == 1) {
-            if ((e = run_zero<T>(fw, nftot * blk, L))) return e;
-            if ((e = run_spread<T>(method, dim, sp.ns, M, kx, ky, kz, cb, fw, geom, kp, sb, maxsub,
-                                   sub_bound, blk, nftot, L)))
-                return e;
-            if ((e = fft(fw, blk))) return e;
-            if ((e = run_deconvolve<T>(dim, ms, nf, fw, fb, kers, blk, L))) return e;
+            tbegin();
+            e = run_zero<T>(fw, nftot * blk, L);
+            tend(kStageGrid);
+            if (e) return e;
+            tbegin();
+            e = run_spread<T>(method, dim, sp.ns, M, kx, ky, kz, cb, fw, geom, kp, sb, maxsub,
+                              sub_bound, blk, nftot, L);
+            tend(kStageSpreadInterp);
+            if (e) return e;
+            tbegin();
+            e = fft(fw, blk);
+            tend(kStageFft);
+            if (e) return e;
+            tbegin();
+            e = run_deconvolve<T>(dim, ms, nf, fw, fb, kers, blk, L);
+            tend(kStageDeconv);
+            if (e) return e;
         } else {
-            if ((e = run_amplify<T>(dim, ms, nf, fw, fb, kers, blk, L))) return e;
-            if ((e = fft(fw, blk))) return e;
-            if ((e = run_interp<T>(method, dim, sp.ns, M, kx, ky, kz, cb, fw, geom, kp, sb, maxsub,
-                                   sub_bound, blk, nftot, L)))
-                return e;
+            tbegin();
+            e = run_amplify<T>(dim, ms, nf, fw, fb, kers, blk, L);
+            tend(kStageDeconv);
+            if (e) return e;
+            tbegin();
+            e = fft(fw, blk);
+            tend(kStageFft);
+            if (e) return e;
+            tbegin();
+            e = run_interp<T>(method, dim, sp.ns, M, kx, ky, kz, cb, fw, geom, kp, sb, maxsub,
+                              sub_bound, blk, nftot, L);
+            tend(kStageSpreadInterp);
+            if (e) return e;
         }
     }
+    return 0;
+}
+
+template <typename T>
+int get_timing(Plan<T> &p, double *ms, int *count)
+{
+    for (int s = 0; s < kNumStages; ++s) {
+        if (ms) ms[s] = 0;
+        if (count) count[s] = 0;
+    }
+    DeviceGuard dg(p.opts.gpu_device_id);
+    if (hipStreamSynchronize(p.L.stream) != hipSuccess) return kErrHip;
+    for (const auto &sp : p.spans) {
+        float t = 0;
+        if (hipEventElapsedTime(&t, sp.a, sp.b) != hipSuccess) return kErrHip;
+        if (ms) ms[sp.stage] += t;
+        if (count) count[sp.stage] += 1;
+    }
+    p.spans.clear();
+    p.ev_used = 0;
     return 0;
 }
 
@@ -577,13 +670,15 @@ int makeplan_impl(int type, int dim, int *nmodes, int iflag, int ntransf, T tol,
     *out = nullptr;
     P *p = new (std::nothrow) P();
     if (!p) return 1;
+    // Any failure frees the plan; a successful plan may still carry the
+    // reference's WARN_EPS_TOO_SMALL (tol clamped to machine precision).
     const int ier = p->makeplan(type, dim, nmodes, iflag, ntransf, tol, maxbatch, opts);
-    if (ier > 1) {
+    if (ier != 0) {
         delete p;
         return ier;
     }
     *out = p;
-    return ier;
+    return p->eps_warning ? 1 : 0;
 }
 
 template <typename P>
@@ -721,6 +816,27 @@ int cufinufftf_get_sort(cufinufftf_plan plan, int *binsize, int *binstart, int *
                         int *subprobstart, int *subprob_to_bin)
 {
     return get_sort_impl(plan, binsize, binstart, idxnupts, subprobstart, subprob_to_bin);
+}
+
+int cufinufft_set_timing(cufinufft_plan plan, int on)
+{
+    if (!plan) return 1;
+    plan->timing = on != 0;
+    return 0;
+}
+int cufinufftf_set_timing(cufinufftf_plan plan, int on)
+{
+    if (!plan) return 1;
+    plan->timing = on != 0;
+    return 0;
+}
+int cufinufft_get_timing(cufinufft_plan plan, double *ms, int *count)
+{
+    return plan ? cfa::get_timing<double>(*plan, ms, count) : 1;
+}
+int cufinufftf_get_timing(cufinufftf_plan plan, double *ms, int *count)
+{
+    return plan ? cfa::get_timing<float>(*plan, ms, count) : 1;
 }
 
 int cufinufft_get_fseries(cufinufft_plan plan, double *k1, double *k2, double *k3)

@@ -90,6 +90,17 @@ int cufinufft_get_sort(cufinufft_plan plan, int *binsize, int *binstart,
 int cufinufftf_get_sort(cufinufftf_plan plan, int *binsize, int *binstart,
                         int *idxnupts, int *subprobstart, int *subprob_to_bin);
 
+/* Stage timer for profiling and tuning: while on, every stage of setpts and
+ * execute is bracketed by HIP events on the plan's stream (no host
+ * synchronisation).  get_timing synchronises the stream, returns per-stage
+ * total milliseconds ms[8] and launch counts count[8] since the last read,
+ * and clears them.  Stages: 0 bin sort, 1 fine-grid zero fill, 2 spread or
+ * interp, 3 FFT, 4 deconvolve or amplify; 5-7 reserved. */
+int cufinufft_set_timing(cufinufft_plan plan, int on);
+int cufinufftf_set_timing(cufinufftf_plan plan, int on);
+int cufinufft_get_timing(cufinufft_plan plan, double *ms, int *count);
+int cufinufftf_get_timing(cufinufftf_plan plan, double *ms, int *count);
+
 /* Fourier-series factors of the plan (host copies of fwkerhalf_d, length
  * nf_d/2+1, FP64 as computed at plan time; NULL skips an axis). */
 int cufinufft_get_fseries(cufinufft_plan plan, double *k1, double *k2, double *k3);
