@@ -123,6 +123,35 @@ def direct_type1(pts, c, N, iflag, kidx, chunk=200_000):
     return out
 
 
+def direct_type1_gpu(dpts, dc, N, iflag, kidx, chunk=1 << 16):
+    """Exact FP64 type-1 sum at flat mode indices kidx, on the GPU with torch
+    (for point counts where the CPU sum would take minutes): phases in FP64,
+    streamed over point chunks."""
+    import torch
+    dev = dpts[0].device
+    sg = 1.0 if iflag >= 0 else -1.0
+    rem = torch.as_tensor(np.asarray(kidx), device=dev)
+    ks = []
+    for d in range(len(dpts)):
+        ks.append((rem % N[d] - N[d] // 2).double())
+        rem = rem // N[d]
+    re = torch.zeros(len(kidx), dtype=torch.float64, device=dev)
+    im = torch.zeros_like(re)
+    M = dpts[0].numel()
+    for s in range(0, M, chunk):
+        e = min(M, s + chunk)
+        ph = ks[0][:, None] * dpts[0][None, s:e].double()
+        for d in range(1, len(dpts)):
+            ph += ks[d][:, None] * dpts[d][None, s:e].double()
+        ph *= sg
+        cr = dc[s:e].real.double()
+        ci = dc[s:e].imag.double()
+        co, si = torch.cos(ph), torch.sin(ph)
+        re += co @ cr - si @ ci
+        im += si @ cr + co @ ci
+    return (re + 1j * im).cpu().numpy()
+
+
 def rel_l2(a, b):
     return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
 
@@ -202,19 +231,26 @@ def run_plan(w, dev, dpts, din, opts, steps, warmup, maxbatch):
                 info=plan.info())
 
 
-def check_accuracy(w, pts, host_in, out, rng):
+def direct_reference(w, pts, host_in, dpts, din, rng):
+    """1000 sampled outputs of the first transform and their exact FP64 values."""
     T = w["ntransf"]
     if w["type"] == 2:
         pidx = rng.choice(w["M"], 1000, replace=False)
         fk0 = host_in[0] if T > 1 else host_in
-        ref = direct_type2(pts, fk0, -1, pidx)
-        got = out[:w["M"]].cpu().numpy()[pidx]
+        return pidx, direct_type2(pts, fk0, -1, pidx)
+    ntot = int(np.prod(w["N"]))
+    kidx = rng.choice(ntot, 1000, replace=False)
+    if w["M"] <= 2_000_000:
+        return kidx, direct_type1(pts, host_in[:w["M"]], w["N"], +1, kidx)
+    return kidx, direct_type1_gpu(dpts, din[:w["M"]], w["N"], +1, kidx)
+
+
+def check_accuracy(w, out, sel, ref):
+    if w["type"] == 2:
+        got = out[:w["M"]].cpu().numpy()[sel]
     else:
-        ntot = int(np.prod(w["N"]))
-        kidx = rng.choice(ntot, 1000, replace=False)
-        ref = direct_type1(pts, host_in[:w["M"]], w["N"], +1, kidx)
         o = out.cpu().numpy()
-        got = (o[0] if T > 1 else o).ravel()[kidx]
+        got = (o[0] if w["ntransf"] > 1 else o).ravel()[sel]
     return rel_l2(got, ref)
 
 
@@ -261,24 +297,6 @@ def cpu_baseline(w, pts, host_in, budget_pts):
 
 
 # ---------------------------------------------------------------- main
-def candidates(w):
-    """Tuner search space for the workload (method x bins x subproblem size)."""
-    dim = w["dim"]
-    c = []
-    if dim == 3:
-        bins = [(16, 16, 2), (8, 8, 8), (8, 8, 4), (16, 8, 4), (4, 4, 4)]
-    elif dim == 2:
-        bins = [(32, 32, 1), (16, 16, 1), (64, 32, 1), (64, 64, 1)]
-    else:
-        bins = [(1024, 1, 1), (512, 1, 1)]
-    for b in bins:
-        for meth in (1, 2):
-            for msub in ((1024,) if meth == 1 else (1024, 256)):
-                c.append(dict(gpu_method=meth, gpu_binsizex=b[0], gpu_binsizey=b[1],
-                              gpu_binsizez=b[2], gpu_maxsubprobsize=msub))
-    return c
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -326,7 +344,9 @@ def main():
     maxbatch = min(T, 8)
     log("[bench] plain plan (reference default options)")
     plain = run_plan(w, dev, dpts, din, {}, a.steps, a.warmup, maxbatch)
-    err_plain = check_accuracy(w, pts, host_in, plain["out"], np.random.default_rng(99))
+    log("[bench] exact FP64 reference at 1000 sampled outputs")
+    sel, ref = direct_reference(w, pts, host_in, dpts, din, np.random.default_rng(99))
+    err_plain = check_accuracy(w, plain["out"], sel, ref)
     log(f"[bench] plain: {plain['t_exec'] * 1e3:.3f} ms/exec, rel_l2 {err_plain:.3e}, "
         f"info {plain['info']}")
     best, best_opts = plain, {}
@@ -336,21 +356,16 @@ def main():
         log(f"[bench] given options {best_opts}")
         best = run_plan(w, dev, dpts, din, best_opts, a.steps, a.warmup, maxbatch)
     elif not a.no_tune:
-        for opts in candidates(w):
-            try:
-                r = run_plan(w, dev, dpts, din, opts, max(2, a.steps // 3), 1, maxbatch)
-            except RuntimeError as e:
-                log(f"[bench]   {opts}: {e}")
-                continue
-            tried.append((opts, r["t_exec"]))
-            log(f"[bench]   {opts}: {r['t_exec'] * 1e3:.3f} ms")
-            if r["t_exec"] < best["t_exec"]:
-                best, best_opts = r, opts
-            del r
-        if best is not plain:
-            log(f"[bench] tuned choice {best_opts}; re-timing")
-            best = run_plan(w, dev, dpts, din, best_opts, a.steps, a.warmup, maxbatch)
-    err_best = check_accuracy(w, pts, host_in, best["out"], np.random.default_rng(99))
+        from cufinufft_amd import autotune
+        log("[bench] autotuning (cufinufft_amd.autotune)")
+        best_opts, tried = autotune.tune(w["type"], tuple(w["N"][::-1]), dpts[::-1], w["tol"],
+                                         dtype=dt, n_trans=T, maxbatchsize=maxbatch,
+                                         use_cache=False, log=log)
+        log(f"[bench] tuned choice {best_opts}; re-timing")
+        best = run_plan(w, dev, dpts, din, best_opts, a.steps, a.warmup, maxbatch)
+        if plain["t_exec"] < best["t_exec"]:
+            best, best_opts = plain, {}
+    err_best = check_accuracy(w, best["out"], sel, ref)
     M = w["M"]
     value = M * T / best["t_exec"]
     st = best["stages"]["spread_interp"]

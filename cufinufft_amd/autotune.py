@@ -1,0 +1,136 @@
+"""Autotuner for the spread / interp method and its geometry.
+
+For a transform shape (type, modes, precision, tol, ntransf) and the caller's
+actual points, time every candidate option set -- method (1 GM-sort, 2 SM
+subproblems), bin size and maximum subproblem size -- on the device and
+return the fastest.  Timing uses the library's HIP-event stage timer around
+the spread / interp kernel plus the whole execute, so it measures exactly the
+kernels the plan will run.  Results are cached in memory per shape key and,
+if ``CUFINUFFT_AMD_TUNE_CACHE`` names a JSON file, across processes.
+
+    from cufinufft_amd import autotune
+    best, table = autotune.tune(1, (256, 256, 256), pts, eps=1e-5, dtype=np.float32)
+    plan = cufinufft(1, (256, 256, 256), eps=1e-5, dtype=np.float32, **best)
+
+The candidate list follows the hardware: spread tiles accumulate in FP64 LDS
+(16 bytes per cell) and interp tiles hold one complex per cell, both capped
+at the 160 KiB LDS of a gfx950 workgroup; plans whose tile does not fit are
+skipped (makeplan reports it).
+"""
+import json
+import os
+import time
+
+import numpy as np
+
+_CACHE = {}
+
+
+def _key(nufft_type, modes, dtype, eps, n_trans, M, upsampfac):
+    return json.dumps([nufft_type, list(modes), np.dtype(dtype).name, float(eps), int(n_trans),
+                       int(M), float(upsampfac)])
+
+
+def candidates(nufft_type, dim):
+    """Option sets to try (cufinufft_opts fields)."""
+    if dim == 3:
+        bins = [(16, 16, 2), (8, 8, 8), (8, 8, 4), (16, 8, 4), (4, 4, 4), (16, 16, 4)]
+    elif dim == 2:
+        bins = [(32, 32, 1), (16, 16, 1), (64, 32, 1), (64, 64, 1), (32, 16, 1)]
+    else:
+        bins = [(1024, 1, 1), (512, 1, 1), (2048, 1, 1)]
+    subs = (256, 1024, 4096)
+    out = []
+    for b in bins:
+        for msub in subs:
+            out.append(dict(gpu_method=2, gpu_binsizex=b[0], gpu_binsizey=b[1],
+                            gpu_binsizez=b[2], gpu_maxsubprobsize=msub))
+    # GM-sort: the bin size only sets the sort order
+    for b in bins[:2]:
+        out.append(dict(gpu_method=1, gpu_binsizex=b[0], gpu_binsizey=b[1], gpu_binsizez=b[2],
+                        gpu_maxsubprobsize=1024))
+    return out
+
+
+def _load_disk():
+    path = os.environ.get("CUFINUFFT_AMD_TUNE_CACHE")
+    if path and os.path.exists(path):
+        try:
+            with open(path) as f:
+                _CACHE.update(json.load(f))
+        except (OSError, ValueError):
+            pass
+
+
+def _save_disk():
+    path = os.environ.get("CUFINUFFT_AMD_TUNE_CACHE")
+    if path:
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(_CACHE, f)
+        os.replace(tmp, path)
+
+
+def tune(nufft_type, modes, pts, eps, dtype=np.float32, n_trans=1, maxbatchsize=1,
+         upsampfac=2.0, reps=2, cands=None, use_cache=True, log=None):
+    """Time candidate options on the given device points.
+
+    modes: numpy-order shape (nZ, nY, nX) as for the ``cufinufft`` class;
+    pts: device arrays in numpy order (as for ``set_pts``).
+    Returns (best_opts, [(opts, exec_ms) ...]).
+    """
+    import torch
+
+    from cufinufft_amd.cufinufft import cufinufft
+    dim = len(modes)
+    M = int(pts[0].numel()) if hasattr(pts[0], "numel") else int(pts[0].size)
+    key = _key(nufft_type, modes, dtype, eps, n_trans, M, upsampfac)
+    if use_cache:
+        if not _CACHE:
+            _load_disk()
+        if key in _CACHE:
+            return dict(_CACHE[key]), []
+    dev = pts[0].device
+    cd = torch.complex64 if np.dtype(dtype) == np.float32 else torch.complex128
+    shape = tuple(modes)
+    g = torch.Generator(device=dev).manual_seed(0)
+    if nufft_type == 1:
+        inp = torch.randn(M * n_trans, dtype=cd, device=dev, generator=g)
+        out = torch.empty(((n_trans,) if n_trans > 1 else ()) + shape, dtype=cd, device=dev)
+        args = (inp, out)
+    else:
+        inp = torch.randn(((n_trans,) if n_trans > 1 else ()) + shape, dtype=cd, device=dev,
+                          generator=g)
+        out = torch.empty(M * n_trans, dtype=cd, device=dev)
+        args = (out, inp)
+    table = []
+    best, best_t = None, float("inf")
+    for opts in (cands or candidates(nufft_type, dim)):
+        try:
+            plan = cufinufft(nufft_type, shape, n_trans=n_trans, eps=eps, dtype=dtype,
+                             maxbatchsize=maxbatchsize, upsampfac=upsampfac, **opts)
+            plan.set_pts(*pts)
+            plan.execute(*args)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                plan.execute(*args)
+            torch.cuda.synchronize()
+            t = (time.perf_counter() - t0) / reps * 1e3
+        except RuntimeError as e:
+            if log:
+                log(f"  skip {opts}: {e}")
+            continue
+        finally:
+            plan = None
+        table.append((opts, t))
+        if log:
+            log(f"  {opts}: {t:.3f} ms")
+        if t < best_t:
+            best, best_t = opts, t
+    if best is None:
+        raise RuntimeError("no candidate option set could be planned")
+    _CACHE[key] = best
+    if use_cache:
+        _save_disk()
+    return dict(best), table

@@ -31,6 +31,38 @@ struct KerParams {
     T hw;    // half width ns/2
 };
 
+// Points as the execute kernels read them, in bin-sorted order: idx[i] is
+// the caller's index of sorted point i (for strengths / outputs) and
+// pos[d*stride + i] its rescaled grid coordinate along axis d (the unrounded
+// double RESCALE value clamped to [-1, nf+1]), written once by setpts so that
+// execute streams positions instead of gathering and rescaling coordinates.
+struct SortedPts {
+    const int *idx;
+    const double *pos;
+    long long stride;
+};
+
+// Workgroups are dispatched round-robin over the XCDs of the device (8 on
+// gfx950, each with its own L2).  Persistent kernels map blockIdx so that each
+// XCD walks a contiguous range of jobs: neighbouring bins, whose halo tiles
+// overlap, then share one L2.  A grid that is a multiple of kNumXcd keeps the
+// mapping exact; any other grid is still correct (all jobs are covered).
+constexpr int kNumXcd = 8;
+
+struct JobRange {
+    long long begin, end, step;
+};
+
+__device__ __forceinline__ JobRange xcd_jobs(long long njobs)
+{
+    const int G = gridDim.x;
+    if (G % kNumXcd != 0) return {(long long)blockIdx.x, njobs, (long long)G};
+    const int xcd = blockIdx.x % kNumXcd;
+    const int per = G / kNumXcd;
+    const long long b = njobs * xcd / kNumXcd, e = njobs * (xcd + 1) / kNumXcd;
+    return {b + blockIdx.x / kNumXcd, e, (long long)per};
+}
+
 // Subproblem bookkeeping written by setpts, read by the SM kernels.
 struct SubprobArgs {
     const int *s2b;       // subproblem -> bin
@@ -177,13 +209,36 @@ k_bin_count(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__
     }
 }
 
-// idxnupts[binstart[b] + rank] = i (CalcInvertofGlobalSortIdx, spreadinterp2d.cu:128-149).
-static __global__ void __launch_bounds__(256)
-k_bin_scatter(int M, const int *__restrict__ binof, const int *__restrict__ rank,
-              const int *__restrict__ binstart, int *__restrict__ idx)
+// idxnupts[binstart[b] + rank] = i (CalcInvertofGlobalSortIdx, spreadinterp2d.cu:
+// 128-149), and the point's rescaled positions into the same sorted slot.
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+k_bin_scatter(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
+              BinGeom g, const int *__restrict__ binof, const int *__restrict__ rank,
+              const int *__restrict__ binstart, int *__restrict__ idx, double *__restrict__ pos,
+              long long stride)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x)
-        idx[binstart[binof[i]] + rank[i]] = i;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+        const int s = binstart[binof[i]] + rank[i];
+        idx[s] = i;
+        pos[s] = tap_pos(rescale_exact(x[i], g.nf[0]), g.nf[0]);
+        if (DIM > 1) pos[stride + s] = tap_pos(rescale_exact(y[i], g.nf[1]), g.nf[1]);
+        if (DIM > 2) pos[2 * stride + s] = tap_pos(rescale_exact(z[i], g.nf[2]), g.nf[2]);
+    }
+}
+
+// Natural order (gpu_sort = 0, TrivialGlobalSortIdx): idx[i] = i and positions.
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+k_natural_order(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
+                BinGeom g, int *__restrict__ idx, double *__restrict__ pos, long long stride)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+        idx[i] = i;
+        pos[i] = tap_pos(rescale_exact(x[i], g.nf[0]), g.nf[0]);
+        if (DIM > 1) pos[stride + i] = tap_pos(rescale_exact(y[i], g.nf[1]), g.nf[1]);
+        if (DIM > 2) pos[2 * stride + i] = tap_pos(rescale_exact(z[i], g.nf[2]), g.nf[2]);
+    }
 }
 
 // Exclusive scans of (binsize, numsubprob = ceil(binsize/maxsub)) in three
@@ -318,33 +373,33 @@ static __global__ void __launch_bounds__(256) k_fill_int(long long n, int v, int
         p[i] = v;
 }
 
-static __global__ void __launch_bounds__(256) k_iota(int M, int *__restrict__ idx)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x)
-        idx[i] = i;
-}
-
 // ===========================================================================
 // execute: spreading (type 1) and interpolation (type 2)
 // ===========================================================================
+
+// Sorted-point chunks of 256 for the point-driven kernels: each XCD takes a
+// contiguous range of the bin-sorted order, so nearby points share its L2.
+constexpr int kPtChunk = 256;
 
 // GM / GM-sort spread: one thread per point, in idxnupts order, global float
 // atomics for each of the ns^DIM taps; all ntr transforms of the batch reuse
 // the kernel values (Spread_{2,3}d_NUptsdriven, spreadinterp2d.cu:16-58).
 template <typename T, int DIM, int NS>
 __global__ void __launch_bounds__(256)
-k_spread_gm(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
-            const int *__restrict__ idx, const cpx<T> *__restrict__ c, cpx<T> *__restrict__ fw,
+k_spread_gm(int M, SortedPts sp, const cpx<T> *__restrict__ c, cpx<T> *__restrict__ fw,
             BinGeom g, KerParams<T> kp, int ntr, long long nftot)
 {
     const int n1 = g.nf[0], n2 = g.nf[1], n3 = g.nf[2];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
-        const int j = idx[i];
+    const JobRange jr = xcd_jobs((M + kPtChunk - 1) / kPtChunk);
+    for (long long ch = jr.begin; ch < jr.end; ch += jr.step) {
+        const int i = (int)(ch * kPtChunk) + (int)threadIdx.x;
+        if (i >= M) continue;
+        const int j = sp.idx[i];
         T k1[NS], k2[NS], k3[NS];
-        const int s1 = taps_global<T, NS>(tap_pos(rescale_exact(x[j], n1), n1), kp, k1);
+        const int s1 = taps_global<T, NS>(sp.pos[i], kp, k1);
         int s2 = 0, s3 = 0;
-        if (DIM > 1) s2 = taps_global<T, NS>(tap_pos(rescale_exact(y[j], n2), n2), kp, k2);
-        if (DIM > 2) s3 = taps_global<T, NS>(tap_pos(rescale_exact(z[j], n3), n3), kp, k3);
+        if (DIM > 1) s2 = taps_global<T, NS>(sp.pos[sp.stride + i], kp, k2);
+        if (DIM > 2) s3 = taps_global<T, NS>(sp.pos[2 * sp.stride + i], kp, k3);
         int ix[NS];
 #pragma unroll
         for (int a = 0; a < NS; ++a) ix[a] = wrap1(s1 + a, n1);
@@ -376,18 +431,20 @@ k_spread_gm(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__
 // (Interp_{2,3}d_NUptsdriven, spreadinterp2d.cu:516-554).
 template <typename T, int DIM, int NS>
 __global__ void __launch_bounds__(256)
-k_interp_gm(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
-            const int *__restrict__ idx, cpx<T> *__restrict__ c, const cpx<T> *__restrict__ fw,
+k_interp_gm(int M, SortedPts sp, cpx<T> *__restrict__ c, const cpx<T> *__restrict__ fw,
             BinGeom g, KerParams<T> kp, int ntr, long long nftot)
 {
     const int n1 = g.nf[0], n2 = g.nf[1], n3 = g.nf[2];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
-        const int j = idx[i];
+    const JobRange jr = xcd_jobs((M + kPtChunk - 1) / kPtChunk);
+    for (long long ch = jr.begin; ch < jr.end; ch += jr.step) {
+        const int i = (int)(ch * kPtChunk) + (int)threadIdx.x;
+        if (i >= M) continue;
+        const int j = sp.idx[i];
         T k1[NS], k2[NS], k3[NS];
-        const int s1 = taps_global<T, NS>(tap_pos(rescale_exact(x[j], n1), n1), kp, k1);
+        const int s1 = taps_global<T, NS>(sp.pos[i], kp, k1);
         int s2 = 0, s3 = 0;
-        if (DIM > 1) s2 = taps_global<T, NS>(tap_pos(rescale_exact(y[j], n2), n2), kp, k2);
-        if (DIM > 2) s3 = taps_global<T, NS>(tap_pos(rescale_exact(z[j], n3), n3), kp, k3);
+        if (DIM > 1) s2 = taps_global<T, NS>(sp.pos[sp.stride + i], kp, k2);
+        if (DIM > 2) s3 = taps_global<T, NS>(sp.pos[2 * sp.stride + i], kp, k3);
         int ix[NS];
 #pragma unroll
         for (int a = 0; a < NS; ++a) ix[a] = wrap1(s1 + a, n1);
@@ -443,54 +500,76 @@ struct Tile {
         o[2] = DIM > 2 ? bz * g.bs[2] - P : 0;
         n = tw[0] * tw[1] * tw[2];
     }
+    // fine-grid offset of tile cell q (periodic wrap of every axis)
+    __device__ __forceinline__ long long grid_offset(int q, const BinGeom &g) const
+    {
+        const int qx = q % tw[0];
+        const int qy = (q / tw[0]) % tw[1];
+        const int qz = q / (tw[0] * tw[1]);
+        const int gx = wrapmod(o[0] + qx, g.nf[0]);
+        const int gy = DIM > 1 ? wrapmod(o[1] + qy, g.nf[1]) : 0;
+        const int gz = DIM > 2 ? wrapmod(o[2] + qz, g.nf[2]) : 0;
+        return ((long long)gz * g.nf[1] + gy) * g.nf[0] + gx;
+    }
 };
+
+struct SubprobJob {
+    int t, b, p0, np;
+};
+
+__device__ __forceinline__ SubprobJob subprob_job(long long job, int total, const SubprobArgs &sa)
+{
+    SubprobJob J;
+    J.t = (int)(job / total);
+    const int s = (int)(job - (long long)J.t * total);
+    J.b = sa.s2b[s];
+    const int kk = s - sa.substart[J.b];
+    J.p0 = sa.binstart[J.b] + kk * sa.maxsub;
+    J.np = min(sa.maxsub, sa.binsize[J.b] - kk * sa.maxsub);
+    return J;
+}
 
 // SM spread (Spread_{2,3}d_Subprob, spreadinterp2d.cu:152-231,
 // spreadinterp3d.cu:285-389): a workgroup takes one (subproblem, transform)
-// job at a time from a grid-stride queue, zeroes an LDS tile (separate
-// re / im planes, so consecutive x taps hit consecutive banks), accumulates
-// every tap of the subproblem's points with LDS float atomics, then flushes
-// the non-zero tile cells to the fine grid with global float atomics (the
-// halo overlaps neighbouring bins).  The job count comes from device memory,
-// so setpts never has to copy it back to the host.
+// job at a time from its XCD's contiguous job range, zeroes an LDS tile
+// (separate re / im planes, so consecutive x taps hit consecutive banks),
+// accumulates every tap of the subproblem's points with LDS atomics, then
+// flushes the non-zero tile cells, row by row (coalesced), to the fine grid
+// with global float atomics (the halo overlaps neighbouring bins).  The job
+// count comes from device memory, so setpts never copies it back.
+//
+// The tile accumulates in FP64 for both plan precisions: on gfx950 the LDS
+// ds_add_f64 sustains about 2.5 lane-ops per CU clock against 0.33 for
+// ds_add_f32 (measured), so the wider tile is 7.6x faster as well as more
+// accurate than FP32 accumulation.
 template <typename T, int DIM, int NS>
 __global__ void __launch_bounds__(256)
-k_spread_sm(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
-            const int *__restrict__ idx, const cpx<T> *__restrict__ c, cpx<T> *__restrict__ fw,
+k_spread_sm(int M, SortedPts sp, const cpx<T> *__restrict__ c, cpx<T> *__restrict__ fw,
             BinGeom g, KerParams<T> kp, SubprobArgs sa, int ntr, long long nftot)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int total = sa.counters[1];
-    const long long njobs = (long long)total * ntr;
-    const int n1 = g.nf[0], n2 = g.nf[1], n3 = g.nf[2];
-    for (long long job = blockIdx.x; job < njobs; job += gridDim.x) {
-        const int t = (int)(job / total);
-        const int s = (int)(job - (long long)t * total);
-        const int b = sa.s2b[s];
-        const int kk = s - sa.substart[b];
-        const int p0 = sa.binstart[b] + kk * sa.maxsub;
-        const int np = min(sa.maxsub, sa.binsize[b] - kk * sa.maxsub);
-        const Tile<DIM, NS> tl(g, b);
-        T *sre = reinterpret_cast<T *>(smem);
-        T *sim = sre + tl.n;
+    const JobRange jr = xcd_jobs((long long)total * ntr);
+    for (long long job = jr.begin; job < jr.end; job += jr.step) {
+        const SubprobJob J = subprob_job(job, total, sa);
+        const Tile<DIM, NS> tl(g, J.b);
+        double *sre = reinterpret_cast<double *>(smem);
+        double *sim = sre + tl.n;
         for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
-            sre[q] = 0;
-            sim[q] = 0;
+            sre[q] = 0.0;
+            sim[q] = 0.0;
         }
         __syncthreads();
-        for (int q = threadIdx.x; q < np; q += blockDim.x) {
-            const int j = idx[p0 + q];
+        for (int q = threadIdx.x; q < J.np; q += blockDim.x) {
+            const int i = J.p0 + q;
+            const int j = sp.idx[i];
             T k1[NS], k2[NS], k3[NS];
-            const int l1 = taps_local<T, NS>(tap_pos(rescale_exact(x[j], n1), n1), tl.o[0],
-                                             tl.tw[0], kp, k1);
+            const int l1 = taps_local<T, NS>(sp.pos[i], tl.o[0], tl.tw[0], kp, k1);
             int l2 = 0, l3 = 0;
-            if (DIM > 1)
-                l2 = taps_local<T, NS>(tap_pos(rescale_exact(y[j], n2), n2), tl.o[1], tl.tw[1],
-                                       kp, k2);
+            if (DIM > 1) l2 = taps_local<T, NS>(sp.pos[sp.stride + i], tl.o[1], tl.tw[1], kp, k2);
             if (DIM > 2)
-                l3 = taps_local<T, NS>(tap_pos(rescale_exact(z[j], n3), n3), tl.o[2], tl.tw[2],
-                                       kp, k3);
-            const cpx<T> cv = c[(long long)t * M + j];
+                l3 = taps_local<T, NS>(sp.pos[2 * sp.stride + i], tl.o[2], tl.tw[2], kp, k3);
+            const cpx<T> cv = c[(long long)J.t * M + j];
 #pragma unroll
             for (int dz = 0; dz < (DIM > 2 ? NS : 1); ++dz) {
                 const T kz = DIM > 2 ? k3[dz] : (T)1;
@@ -501,24 +580,18 @@ k_spread_sm(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__
                     const int row = ((l3 + dz) * tl.tw[1] + (l2 + dy)) * tl.tw[0] + l1;
 #pragma unroll
                     for (int a = 0; a < NS; ++a) {
-                        ladd(&sre[row + a], cr * k1[a]);
-                        ladd(&sim[row + a], ci * k1[a]);
+                        ladd(&sre[row + a], (double)(cr * k1[a]));
+                        ladd(&sim[row + a], (double)(ci * k1[a]));
                     }
                 }
             }
         }
         __syncthreads();
-        cpx<T> *f = fw + (long long)t * nftot;
+        cpx<T> *f = fw + (long long)J.t * nftot;
         for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
-            const T vr = sre[q], vi = sim[q];
+            const T vr = (T)sre[q], vi = (T)sim[q];
             if (vr != (T)0 || vi != (T)0) {
-                const int qx = q % tl.tw[0];
-                const int qy = (q / tl.tw[0]) % tl.tw[1];
-                const int qz = q / (tl.tw[0] * tl.tw[1]);
-                const int gx = wrapmod(tl.o[0] + qx, n1);
-                const int gy = DIM > 1 ? wrapmod(tl.o[1] + qy, n2) : 0;
-                const int gz = DIM > 2 ? wrapmod(tl.o[2] + qz, n3) : 0;
-                cpx<T> *p = f + ((long long)gz * n2 + gy) * n1 + gx;
+                cpx<T> *p = f + tl.grid_offset(q, g);
                 gadd(&p->x, vr);
                 gadd(&p->y, vi);
             }
@@ -528,53 +601,32 @@ k_spread_sm(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__
 }
 
 // SM interp (Interp_{2,3}d_Subprob, spreadinterp2d.cu:596-669): load the
-// padded tile of the subproblem's bin into LDS once, then each thread gathers
-// the taps of its points from LDS.
+// padded tile of the subproblem's bin into LDS once (complex cells, so one
+// tap is one LDS read), then each thread gathers the taps of its points.
 template <typename T, int DIM, int NS>
 __global__ void __launch_bounds__(256)
-k_interp_sm(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
-            const int *__restrict__ idx, cpx<T> *__restrict__ c, const cpx<T> *__restrict__ fw,
+k_interp_sm(int M, SortedPts sp, cpx<T> *__restrict__ c, const cpx<T> *__restrict__ fw,
             BinGeom g, KerParams<T> kp, SubprobArgs sa, int ntr, long long nftot)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int total = sa.counters[1];
-    const long long njobs = (long long)total * ntr;
-    const int n1 = g.nf[0], n2 = g.nf[1], n3 = g.nf[2];
-    for (long long job = blockIdx.x; job < njobs; job += gridDim.x) {
-        const int t = (int)(job / total);
-        const int s = (int)(job - (long long)t * total);
-        const int b = sa.s2b[s];
-        const int kk = s - sa.substart[b];
-        const int p0 = sa.binstart[b] + kk * sa.maxsub;
-        const int np = min(sa.maxsub, sa.binsize[b] - kk * sa.maxsub);
-        const Tile<DIM, NS> tl(g, b);
-        T *sre = reinterpret_cast<T *>(smem);
-        T *sim = sre + tl.n;
-        const cpx<T> *f = fw + (long long)t * nftot;
-        for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
-            const int qx = q % tl.tw[0];
-            const int qy = (q / tl.tw[0]) % tl.tw[1];
-            const int qz = q / (tl.tw[0] * tl.tw[1]);
-            const int gx = wrapmod(tl.o[0] + qx, n1);
-            const int gy = DIM > 1 ? wrapmod(tl.o[1] + qy, n2) : 0;
-            const int gz = DIM > 2 ? wrapmod(tl.o[2] + qz, n3) : 0;
-            const cpx<T> v = f[((long long)gz * n2 + gy) * n1 + gx];
-            sre[q] = v.x;
-            sim[q] = v.y;
-        }
+    const JobRange jr = xcd_jobs((long long)total * ntr);
+    for (long long job = jr.begin; job < jr.end; job += jr.step) {
+        const SubprobJob J = subprob_job(job, total, sa);
+        const Tile<DIM, NS> tl(g, J.b);
+        cpx<T> *tile = reinterpret_cast<cpx<T> *>(smem);
+        const cpx<T> *f = fw + (long long)J.t * nftot;
+        for (int q = threadIdx.x; q < tl.n; q += blockDim.x) tile[q] = f[tl.grid_offset(q, g)];
         __syncthreads();
-        for (int q = threadIdx.x; q < np; q += blockDim.x) {
-            const int j = idx[p0 + q];
+        for (int q = threadIdx.x; q < J.np; q += blockDim.x) {
+            const int i = J.p0 + q;
+            const int j = sp.idx[i];
             T k1[NS], k2[NS], k3[NS];
-            const int l1 = taps_local<T, NS>(tap_pos(rescale_exact(x[j], n1), n1), tl.o[0],
-                                             tl.tw[0], kp, k1);
+            const int l1 = taps_local<T, NS>(sp.pos[i], tl.o[0], tl.tw[0], kp, k1);
             int l2 = 0, l3 = 0;
-            if (DIM > 1)
-                l2 = taps_local<T, NS>(tap_pos(rescale_exact(y[j], n2), n2), tl.o[1], tl.tw[1],
-                                       kp, k2);
+            if (DIM > 1) l2 = taps_local<T, NS>(sp.pos[sp.stride + i], tl.o[1], tl.tw[1], kp, k2);
             if (DIM > 2)
-                l3 = taps_local<T, NS>(tap_pos(rescale_exact(z[j], n3), n3), tl.o[2], tl.tw[2],
-                                       kp, k3);
+                l3 = taps_local<T, NS>(sp.pos[2 * sp.stride + i], tl.o[2], tl.tw[2], kp, k3);
             T ar = 0, ai = 0;
 #pragma unroll
             for (int dz = 0; dz < (DIM > 2 ? NS : 1); ++dz) {
@@ -586,8 +638,9 @@ k_interp_sm(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__
                     T rr = 0, ri = 0;
 #pragma unroll
                     for (int a = 0; a < NS; ++a) {
-                        rr += sre[row + a] * k1[a];
-                        ri += sim[row + a] * k1[a];
+                        const cpx<T> v = tile[row + a];
+                        rr += v.x * k1[a];
+                        ri += v.y * k1[a];
                     }
                     ar += rr * kyz;
                     ai += ri * kyz;
@@ -596,12 +649,11 @@ k_interp_sm(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__
             cpx<T> o;
             o.x = ar;
             o.y = ai;
-            c[(long long)t * M + j] = o;
+            c[(long long)J.t * M + j] = o;
         }
         __syncthreads();
     }
 }
-
 // ===========================================================================
 // execute: deconvolve (type 1) / amplify (type 2), one fused pass each
 // ===========================================================================

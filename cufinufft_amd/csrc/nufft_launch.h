@@ -18,14 +18,16 @@ constexpr size_t kMaxLdsBytes = 160 * 1024;  // LDS of one gfx950 workgroup
 
 // Device arrays written by setpts.
 struct SortBufs {
-    int *binsize = nullptr;   // [nbins]
-    int *binstart = nullptr;  // [nbins]
-    int *substart = nullptr;  // [nbins + 1]
-    int *s2b = nullptr;       // [nbins + ceil(M / maxsub)] upper bound
-    int *rank = nullptr;      // [M]
-    int *binof = nullptr;     // [M]
-    int *idx = nullptr;       // [M]
-    int *counters = nullptr;  // [2]: points, subproblems
+    int *binsize = nullptr;     // [nbins]
+    int *binstart = nullptr;    // [nbins]
+    int *substart = nullptr;    // [nbins + 1]
+    int *s2b = nullptr;         // [nbins + ceil(M / maxsub)] upper bound
+    int *rank = nullptr;        // [M]
+    int *binof = nullptr;       // [M]
+    int *idx = nullptr;         // [M] caller index of each sorted point
+    double *pos = nullptr;      // [dim][pos_stride] sorted rescaled positions
+    long long pos_stride = 0;   // capacity in points of each pos plane
+    int *counters = nullptr;    // [2]: points, subproblems
     void *scan_part = nullptr;  // int2[ceil(nbins / chunk)]
 };
 
@@ -34,26 +36,28 @@ struct Launch {
     int ncu = 256;
 };
 
-// Bin sort of M points (counts, ranks, scans, scatter, subproblem map).
+// Bin sort of M points (counts, ranks, scans, scatter of indices and
+// positions into sorted order, subproblem map).
 template <typename T>
 int run_bin_sort(int dim, const BinGeom &g, long long nbins, int M, const T *x, const T *y,
                  const T *z, int maxsub, bool need_subprob, SortBufs &sb, const Launch &L);
-// Identity order (gpu_sort = 0).
-int run_iota(int M, int *idx, const Launch &L);
-
-// Spread / interp of ntr transforms.  method 1 = GM(-sort), 2 = SM.
+// Natural order (gpu_sort = 0): identity index and positions.
 template <typename T>
-int run_spread(int method, int dim, int ns, int M, const T *x, const T *y, const T *z,
-               const cpx<T> *c, cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp,
-               const SortBufs &sb, int maxsub, long long sub_bound, int ntr, long long nftot,
-               const Launch &L);
-template <typename T>
-int run_interp(int method, int dim, int ns, int M, const T *x, const T *y, const T *z,
-               cpx<T> *c, const cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp,
-               const SortBufs &sb, int maxsub, long long sub_bound, int ntr, long long nftot,
-               const Launch &L);
+int run_natural_order(int dim, const BinGeom &g, int M, const T *x, const T *y, const T *z,
+                      SortBufs &sb, const Launch &L);
 
-// Bytes of LDS the SM kernels need for a tile (re and im planes).
+// Spread / interp of ntr transforms over the sorted points of sb.
+// method 1 = GM(-sort), 2 = SM.
+template <typename T>
+int run_spread(int method, int dim, int ns, int M, const SortBufs &sb, const cpx<T> *c,
+               cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp, int maxsub,
+               long long sub_bound, int ntr, long long nftot, const Launch &L);
+template <typename T>
+int run_interp(int method, int dim, int ns, int M, const SortBufs &sb, cpx<T> *c,
+               const cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp, int maxsub,
+               long long sub_bound, int ntr, long long nftot, const Launch &L);
+
+// Bytes of LDS the SM kernels need for a tile (re and im of every cell).
 size_t sm_tile_bytes(int dim, int ns, const int *bs, size_t elem);
 
 template <typename T>
@@ -67,12 +71,12 @@ int run_zero(cpx<T> *p, long long n, const Launch &L);
 
 // Per-(T, DIM) spread/interp launchers (one translation unit each).
 template <typename T, int DIM>
-int spread_dim(int method, int ns, int M, const T *x, const T *y, const T *z, const cpx<T> *c,
-               cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp, const SortBufs &sb,
-               int maxsub, long long sub_bound, int ntr, long long nftot, const Launch &L);
+int spread_dim(int method, int ns, int M, const SortBufs &sb, const cpx<T> *c, cpx<T> *fw,
+               const BinGeom &g, const KerParams<T> &kp, int maxsub, long long sub_bound,
+               int ntr, long long nftot, const Launch &L);
 template <typename T, int DIM>
-int interp_dim(int method, int ns, int M, const T *x, const T *y, const T *z, cpx<T> *c,
-               const cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp, const SortBufs &sb,
-               int maxsub, long long sub_bound, int ntr, long long nftot, const Launch &L);
+int interp_dim(int method, int ns, int M, const SortBufs &sb, cpx<T> *c, const cpx<T> *fw,
+               const BinGeom &g, const KerParams<T> &kp, int maxsub, long long sub_bound,
+               int ntr, long long nftot, const Launch &L);
 
 }  // namespace cfa

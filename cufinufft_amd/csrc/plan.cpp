@@ -335,6 +335,7 @@ struct Plan {
         dfree(sb.rank);
         dfree(sb.binof);
         dfree(sb.idx);
+        dfree(sb.pos);
         dfree(sb.counters);
         if (sb.scan_part) hipFree(sb.scan_part);
         sb.scan_part = nullptr;
@@ -467,13 +468,14 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
         if (nftot > 2147483647LL) return kErrNdataNotValid;
     }
 
-    // Method: 0 picks SM for type 1 when the tile fits the LDS, GM-sort otherwise.
+    // Method: 0 picks SM when the tile fits the LDS, GM-sort otherwise.  The
+    // spread tile accumulates in FP64 whatever T is; the interp tile holds T.
     const int bsv[3] = {opts.gpu_binsizex, opts.gpu_binsizey, opts.gpu_binsizez};
+    const size_t tile_bytes =
+        sm_tile_bytes(dim, sp.ns, bsv, type == 1 ? sizeof(double) : sizeof(T));
     method = opts.gpu_method;
-    if (method == 0) {
-        method = (type == 1 && sm_tile_bytes(dim, sp.ns, bsv, sizeof(T)) <= kMaxLdsBytes) ? 2 : 1;
-    }
-    if (method == 2 && sm_tile_bytes(dim, sp.ns, bsv, sizeof(T)) > kMaxLdsBytes) return kErrLds;
+    if (method == 0) method = tile_bytes <= kMaxLdsBytes ? 2 : 1;
+    if (method == 2 && tile_bytes > kMaxLdsBytes) return kErrLds;
 
     for (int d = 0; d < 3; ++d) {
         geom.nf[d] = nf[d];
@@ -531,11 +533,14 @@ int Plan<T>::setpts(int M_, const T *x, const T *y, const T *z)
         dfree(sb.rank);
         dfree(sb.binof);
         dfree(sb.idx);
-        if (dmalloc(&sb.rank, M) || dmalloc(&sb.binof, M) || dmalloc(&sb.idx, M)) {
+        dfree(sb.pos);
+        if (dmalloc(&sb.rank, M) || dmalloc(&sb.binof, M) || dmalloc(&sb.idx, M) ||
+            dmalloc(&sb.pos, (size_t)M * dim)) {
             cap_M = -1;
             return kErrHip;
         }
         cap_M = M;
+        sb.pos_stride = M;
     }
     const int maxsub = opts.gpu_maxsubprobsize;
     sub_bound = nbins + (M + (long long)maxsub - 1) / maxsub;
@@ -551,7 +556,7 @@ int Plan<T>::setpts(int M_, const T *x, const T *y, const T *z)
     int e;
     tbegin();
     if (method == 1 && !opts.gpu_sort) {
-        e = run_iota(M, sb.idx, L);
+        e = run_natural_order<T>(dim, geom, M, kx, ky, kz, sb, L);
     } else {
         e = run_bin_sort<T>(dim, geom, nbins, M, kx, ky, kz, maxsub, need_sub, sb, L);
     }
@@ -580,11 +585,11 @@ int Plan<T>::execute(cpx<T> *c, cpx<T> *fk)
             cpx<T> *fb = fk + (long long)b0 * nftot;
             if (type == 1) {
                 if ((e = run_zero<T>(fb, nftot * blk, L))) return e;
-                if ((e = run_spread<T>(method, dim, sp.ns, M, kx, ky, kz, cb, fb, geom, kp, sb,
+                if ((e = run_spread<T>(method, dim, sp.ns, M, sb, cb, fb, geom, kp,
                                        maxsub, sub_bound, blk, nftot, L)))
                     return e;
             } else {
-                if ((e = run_interp<T>(method, dim, sp.ns, M, kx, ky, kz, cb, fb, geom, kp, sb,
+                if ((e = run_interp<T>(method, dim, sp.ns, M, sb, cb, fb, geom, kp,
                                        maxsub, sub_bound, blk, nftot, L)))
                     return e;
             }
@@ -601,7 +606,7 @@ int Plan<T>::execute(cpx<T> *c, cpx<T> *fk)
             tend(kStageGrid);
             if (e) return e;
             tbegin();
-            e = run_spread<T>(method, dim, sp.ns, M, kx, ky, kz, cb, fw, geom, kp, sb, maxsub,
+            e = run_spread<T>(method, dim, sp.ns, M, sb, cb, fw, geom, kp, maxsub,
                               sub_bound, blk, nftot, L);
             tend(kStageSpreadInterp);
             if (e) return e;
@@ -623,7 +628,7 @@ int Plan<T>::execute(cpx<T> *c, cpx<T> *fk)
             tend(kStageFft);
             if (e) return e;
             tbegin();
-            e = run_interp<T>(method, dim, sp.ns, M, kx, ky, kz, cb, fw, geom, kp, sb, maxsub,
+            e = run_interp<T>(method, dim, sp.ns, M, sb, cb, fw, geom, kp, maxsub,
                               sub_bound, blk, nftot, L);
             tend(kStageSpreadInterp);
             if (e) return e;

@@ -7,19 +7,23 @@ namespace cfa {
 
 namespace {
 
+// Point-driven kernels: enough 256-point chunks to fill the device a few
+// times over, rounded to whole XCD groups.
 inline int grid_for_points(int M, int ncu)
 {
-    long long b = (M + 255) / 256;
-    long long cap = (long long)ncu * 32;
-    return (int)(b < 1 ? 1 : (b > cap ? cap : b));
+    long long chunks = (M + kPtChunk - 1) / kPtChunk;
+    long long cap = (long long)ncu * 16;
+    long long g = chunks < cap ? chunks : cap;
+    if (g >= kNumXcd) g -= g % kNumXcd;
+    return (int)(g < 1 ? 1 : g);
 }
 
 // Persistent grid for a subproblem kernel: as many workgroups as fit on the
-// device at the tile's LDS footprint, never more than there are jobs.
+// device at the tile's LDS footprint, never more than there are jobs, rounded
+// to whole XCD groups.
 template <typename K>
 int sm_grid(K kern, size_t lds, long long jobs, int ncu, int *grid)
 {
-    static_assert(sizeof(K) > 0, "");
     if (lds > 64 * 1024) {
         if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds) != hipSuccess)
@@ -31,39 +35,42 @@ int sm_grid(K kern, size_t lds, long long jobs, int ncu, int *grid)
         return kErrLds;
     long long g = (long long)occ * ncu;
     if (g > jobs) g = jobs;
+    if (g >= kNumXcd) g -= g % kNumXcd;
     *grid = (int)(g < 1 ? 1 : g);
     return 0;
 }
 
+inline SortedPts sorted_pts(const SortBufs &sb) { return SortedPts{sb.idx, sb.pos, sb.pos_stride}; }
+
 template <typename T, int DIM, int NS>
-int spread_ns(int method, int M, const T *x, const T *y, const T *z, const cpx<T> *c, cpx<T> *fw,
-              const BinGeom &g, const KerParams<T> &kp, const SortBufs &sb, int maxsub,
-              long long sub_bound, int ntr, long long nftot, const Launch &L)
+int spread_ns(int method, int M, const SortBufs &sb, const cpx<T> *c, cpx<T> *fw,
+              const BinGeom &g, const KerParams<T> &kp, int maxsub, long long sub_bound, int ntr,
+              long long nftot, const Launch &L)
 {
     if (method == 1) {
         k_spread_gm<T, DIM, NS><<<grid_for_points(M, L.ncu), 256, 0, L.stream>>>(
-            M, x, y, z, sb.idx, c, fw, g, kp, ntr, nftot);
+            M, sorted_pts(sb), c, fw, g, kp, ntr, nftot);
     } else {
-        const size_t lds = sm_tile_bytes(DIM, NS, g.bs, sizeof(T));
+        const size_t lds = sm_tile_bytes(DIM, NS, g.bs, sizeof(double));  // FP64 accumulation
         if (lds > kMaxLdsBytes) return kErrLds;
         int grid = 0;
         int e = sm_grid(k_spread_sm<T, DIM, NS>, lds, sub_bound * ntr, L.ncu, &grid);
         if (e) return e;
         SubprobArgs sa{sb.s2b, sb.substart, sb.binstart, sb.binsize, sb.counters, maxsub};
-        k_spread_sm<T, DIM, NS><<<grid, 256, lds, L.stream>>>(M, x, y, z, sb.idx, c, fw, g, kp,
-                                                              sa, ntr, nftot);
+        k_spread_sm<T, DIM, NS><<<grid, 256, lds, L.stream>>>(M, sorted_pts(sb), c, fw, g, kp, sa,
+                                                              ntr, nftot);
     }
     return hipGetLastError() == hipSuccess ? 0 : kErrHip;
 }
 
 template <typename T, int DIM, int NS>
-int interp_ns(int method, int M, const T *x, const T *y, const T *z, cpx<T> *c,
-              const cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp, const SortBufs &sb,
-              int maxsub, long long sub_bound, int ntr, long long nftot, const Launch &L)
+int interp_ns(int method, int M, const SortBufs &sb, cpx<T> *c, const cpx<T> *fw,
+              const BinGeom &g, const KerParams<T> &kp, int maxsub, long long sub_bound, int ntr,
+              long long nftot, const Launch &L)
 {
     if (method == 1) {
         k_interp_gm<T, DIM, NS><<<grid_for_points(M, L.ncu), 256, 0, L.stream>>>(
-            M, x, y, z, sb.idx, c, fw, g, kp, ntr, nftot);
+            M, sorted_pts(sb), c, fw, g, kp, ntr, nftot);
     } else {
         const size_t lds = sm_tile_bytes(DIM, NS, g.bs, sizeof(T));
         if (lds > kMaxLdsBytes) return kErrLds;
@@ -71,8 +78,8 @@ int interp_ns(int method, int M, const T *x, const T *y, const T *z, cpx<T> *c,
         int e = sm_grid(k_interp_sm<T, DIM, NS>, lds, sub_bound * ntr, L.ncu, &grid);
         if (e) return e;
         SubprobArgs sa{sb.s2b, sb.substart, sb.binstart, sb.binsize, sb.counters, maxsub};
-        k_interp_sm<T, DIM, NS><<<grid, 256, lds, L.stream>>>(M, x, y, z, sb.idx, c, fw, g, kp,
-                                                              sa, ntr, nftot);
+        k_interp_sm<T, DIM, NS><<<grid, 256, lds, L.stream>>>(M, sorted_pts(sb), c, fw, g, kp, sa,
+                                                              ntr, nftot);
     }
     return hipGetLastError() == hipSuccess ? 0 : kErrHip;
 }
@@ -83,15 +90,15 @@ int interp_ns(int method, int M, const T *x, const T *y, const T *z, cpx<T> *c,
     F(2) F(3) F(4) F(5) F(6) F(7) F(8) F(9) F(10) F(11) F(12) F(13) F(14) F(15) F(16)
 
 template <typename T, int DIM>
-int spread_dim(int method, int ns, int M, const T *x, const T *y, const T *z, const cpx<T> *c,
-               cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp, const SortBufs &sb,
-               int maxsub, long long sub_bound, int ntr, long long nftot, const Launch &L)
+int spread_dim(int method, int ns, int M, const SortBufs &sb, const cpx<T> *c, cpx<T> *fw,
+               const BinGeom &g, const KerParams<T> &kp, int maxsub, long long sub_bound,
+               int ntr, long long nftot, const Launch &L)
 {
     switch (ns) {
 #define CFA_CASE(N)                                                                          \
     case N:                                                                                  \
-        return spread_ns<T, DIM, N>(method, M, x, y, z, c, fw, g, kp, sb, maxsub, sub_bound, \
-                                    ntr, nftot, L);
+        return spread_ns<T, DIM, N>(method, M, sb, c, fw, g, kp, maxsub, sub_bound, ntr, nftot, \
+                                    L);
         CFA_NS_CASES(CFA_CASE)
 #undef CFA_CASE
     }
@@ -99,15 +106,15 @@ int spread_dim(int method, int ns, int M, const T *x, const T *y, const T *z, co
 }
 
 template <typename T, int DIM>
-int interp_dim(int method, int ns, int M, const T *x, const T *y, const T *z, cpx<T> *c,
-               const cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp, const SortBufs &sb,
-               int maxsub, long long sub_bound, int ntr, long long nftot, const Launch &L)
+int interp_dim(int method, int ns, int M, const SortBufs &sb, cpx<T> *c, const cpx<T> *fw,
+               const BinGeom &g, const KerParams<T> &kp, int maxsub, long long sub_bound,
+               int ntr, long long nftot, const Launch &L)
 {
     switch (ns) {
 #define CFA_CASE(N)                                                                          \
     case N:                                                                                  \
-        return interp_ns<T, DIM, N>(method, M, x, y, z, c, fw, g, kp, sb, maxsub, sub_bound, \
-                                    ntr, nftot, L);
+        return interp_ns<T, DIM, N>(method, M, sb, c, fw, g, kp, maxsub, sub_bound, ntr, nftot, \
+                                    L);
         CFA_NS_CASES(CFA_CASE)
 #undef CFA_CASE
     }
@@ -117,13 +124,11 @@ int interp_dim(int method, int ns, int M, const T *x, const T *y, const T *z, cp
 }  // namespace cfa
 
 #define CFA_INSTANTIATE_SPREADINTERP(T, DIM)                                                  \
-    template int cfa::spread_dim<T, DIM>(int, int, int, const T *, const T *, const T *,     \
+    template int cfa::spread_dim<T, DIM>(int, int, int, const cfa::SortBufs &,               \
                                          const cfa::cpx<T> *, cfa::cpx<T> *,                 \
                                          const cfa::BinGeom &, const cfa::KerParams<T> &,    \
-                                         const cfa::SortBufs &, int, long long, int,         \
-                                         long long, const cfa::Launch &);                    \
-    template int cfa::interp_dim<T, DIM>(int, int, int, const T *, const T *, const T *,     \
-                                         cfa::cpx<T> *, const cfa::cpx<T> *,                 \
-                                         const cfa::BinGeom &, const cfa::KerParams<T> &,    \
-                                         const cfa::SortBufs &, int, long long, int,         \
+                                         int, long long, int, long long, const cfa::Launch &); \
+    template int cfa::interp_dim<T, DIM>(int, int, int, const cfa::SortBufs &, cfa::cpx<T> *, \
+                                         const cfa::cpx<T> *, const cfa::BinGeom &,          \
+                                         const cfa::KerParams<T> &, int, long long, int,     \
                                          long long, const cfa::Launch &);

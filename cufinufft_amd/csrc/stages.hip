@@ -50,52 +50,64 @@ int run_bin_sort(int dim, const BinGeom &g, long long nbins, int M, const T *x, 
                                                    sb.substart + nbins);
     k_scan_apply<<<(unsigned)nparts, kScanThreads, 0, L.stream>>>(sb.binsize, nbins, maxsub,
                                                                   part, sb.binstart, sb.substart);
-    if (M > 0)
-        k_bin_scatter<<<grid_1d(M, L.ncu), 256, 0, L.stream>>>(M, sb.binof, sb.rank,
-                                                               sb.binstart, sb.idx);
+    if (M > 0) {
+        const int gp = grid_1d(M, L.ncu);
+        const long long st = sb.pos_stride;
+        if (dim == 1)
+            k_bin_scatter<T, 1><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.binof, sb.rank,
+                                                          sb.binstart, sb.idx, sb.pos, st);
+        else if (dim == 2)
+            k_bin_scatter<T, 2><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.binof, sb.rank,
+                                                          sb.binstart, sb.idx, sb.pos, st);
+        else
+            k_bin_scatter<T, 3><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.binof, sb.rank,
+                                                          sb.binstart, sb.idx, sb.pos, st);
+    }
     if (need_subprob)
         k_map_subprob<<<grid_1d(nbins, L.ncu), 256, 0, L.stream>>>(nbins, sb.substart, sb.s2b);
     return ok_or_err();
 }
 
-int run_iota(int M, int *idx, const Launch &L)
+template <typename T>
+int run_natural_order(int dim, const BinGeom &g, int M, const T *x, const T *y, const T *z,
+                      SortBufs &sb, const Launch &L)
 {
-    if (M > 0) k_iota<<<grid_1d(M, L.ncu), 256, 0, L.stream>>>(M, idx);
+    if (M == 0) return 0;
+    const int gp = grid_1d(M, L.ncu);
+    const long long st = sb.pos_stride;
+    if (dim == 1)
+        k_natural_order<T, 1><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.idx, sb.pos, st);
+    else if (dim == 2)
+        k_natural_order<T, 2><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.idx, sb.pos, st);
+    else
+        k_natural_order<T, 3><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.idx, sb.pos, st);
     return ok_or_err();
 }
 
 template <typename T>
-int run_spread(int method, int dim, int ns, int M, const T *x, const T *y, const T *z,
-               const cpx<T> *c, cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp,
-               const SortBufs &sb, int maxsub, long long sub_bound, int ntr, long long nftot,
-               const Launch &L)
+int run_spread(int method, int dim, int ns, int M, const SortBufs &sb, const cpx<T> *c,
+               cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp, int maxsub,
+               long long sub_bound, int ntr, long long nftot, const Launch &L)
 {
     if (M == 0) return 0;
     if (dim == 1)
-        return spread_dim<T, 1>(method, ns, M, x, y, z, c, fw, g, kp, sb, maxsub, sub_bound,
-                                ntr, nftot, L);
+        return spread_dim<T, 1>(method, ns, M, sb, c, fw, g, kp, maxsub, sub_bound, ntr, nftot, L);
     if (dim == 2)
-        return spread_dim<T, 2>(method, ns, M, x, y, z, c, fw, g, kp, sb, maxsub, sub_bound,
-                                ntr, nftot, L);
-    return spread_dim<T, 3>(method, ns, M, x, y, z, c, fw, g, kp, sb, maxsub, sub_bound, ntr,
-                            nftot, L);
+        return spread_dim<T, 2>(method, ns, M, sb, c, fw, g, kp, maxsub, sub_bound, ntr, nftot, L);
+    return spread_dim<T, 3>(method, ns, M, sb, c, fw, g, kp, maxsub, sub_bound, ntr, nftot, L);
 }
 
 template <typename T>
-int run_interp(int method, int dim, int ns, int M, const T *x, const T *y, const T *z,
-               cpx<T> *c, const cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp,
-               const SortBufs &sb, int maxsub, long long sub_bound, int ntr, long long nftot,
-               const Launch &L)
+int run_interp(int method, int dim, int ns, int M, const SortBufs &sb, cpx<T> *c,
+               const cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp, int maxsub,
+               long long sub_bound, int ntr, long long nftot, const Launch &L)
 {
     if (M == 0) return 0;
     if (dim == 1)
-        return interp_dim<T, 1>(method, ns, M, x, y, z, c, fw, g, kp, sb, maxsub, sub_bound,
-                                ntr, nftot, L);
+        return interp_dim<T, 1>(method, ns, M, sb, c, fw, g, kp, maxsub, sub_bound, ntr, nftot, L);
     if (dim == 2)
-        return interp_dim<T, 2>(method, ns, M, x, y, z, c, fw, g, kp, sb, maxsub, sub_bound,
-                                ntr, nftot, L);
-    return interp_dim<T, 3>(method, ns, M, x, y, z, c, fw, g, kp, sb, maxsub, sub_bound, ntr,
-                            nftot, L);
+        return interp_dim<T, 2>(method, ns, M, sb, c, fw, g, kp, maxsub, sub_bound, ntr, nftot, L);
+    return interp_dim<T, 3>(method, ns, M, sb, c, fw, g, kp, maxsub, sub_bound, ntr, nftot, L);
 }
 
 template <typename T>
@@ -149,14 +161,14 @@ int run_zero(cpx<T> *p, long long n, const Launch &L)
 #define CFA_INST(T)                                                                          \
     template int run_bin_sort<T>(int, const BinGeom &, long long, int, const T *, const T *, \
                                  const T *, int, bool, SortBufs &, const Launch &);          \
-    template int run_spread<T>(int, int, int, int, const T *, const T *, const T *,          \
-                               const cpx<T> *, cpx<T> *, const BinGeom &,                    \
-                               const KerParams<T> &, const SortBufs &, int, long long, int,  \
-                               long long, const Launch &);                                   \
-    template int run_interp<T>(int, int, int, int, const T *, const T *, const T *, cpx<T> *, \
-                               const cpx<T> *, const BinGeom &, const KerParams<T> &,        \
-                               const SortBufs &, int, long long, int, long long,             \
-                               const Launch &);                                              \
+    template int run_natural_order<T>(int, const BinGeom &, int, const T *, const T *,       \
+                                      const T *, SortBufs &, const Launch &);                \
+    template int run_spread<T>(int, int, int, int, const SortBufs &, const cpx<T> *,         \
+                               cpx<T> *, const BinGeom &, const KerParams<T> &, int,         \
+                               long long, int, long long, const Launch &);                   \
+    template int run_interp<T>(int, int, int, int, const SortBufs &, cpx<T> *,               \
+                               const cpx<T> *, const BinGeom &, const KerParams<T> &, int,   \
+                               long long, int, long long, const Launch &);                   \
     template int run_deconvolve<T>(int, const int *, const int *, const cpx<T> *, cpx<T> *,  \
                                    const T *const *, int, const Launch &);                   \
     template int run_amplify<T>(int, const int *, const int *, cpx<T> *, const cpx<T> *,     \
