@@ -151,6 +151,27 @@ _get_timingf = lib.cufinufftf_get_timing
 _get_timingf.argtypes = [c_void_p, c_void_p, c_void_p]
 _get_timingf.restype = c_int
 
+_exec_grad = lib.cufinufft_execute_grad
+_exec_grad.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
+_exec_grad.restype = c_int
+_exec_gradf = lib.cufinufftf_execute_grad
+_exec_gradf.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
+_exec_gradf.restype = c_int
+
+for _suffix in ("", "f"):
+    _f = getattr(lib, f"cufinufft{_suffix}_group_make")
+    _f.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_void_p]
+    _f.restype = c_int
+    _f = getattr(lib, f"cufinufft{_suffix}_group_execute")
+    _f.argtypes = [c_void_p, c_int]
+    _f.restype = c_int
+    _f = getattr(lib, f"cufinufft{_suffix}_group_setstream")
+    _f.argtypes = [c_void_p, c_void_p]
+    _f.restype = c_int
+    _f = getattr(lib, f"cufinufft{_suffix}_group_destroy")
+    _f.argtypes = [c_void_p]
+    _f.restype = c_int
+
 STAGES = ("sort", "grid_zero", "spread_interp", "fft", "deconv_amplify")
 
 EXPORTED = [
@@ -165,4 +186,9 @@ EXPORTED = [
     "cufinufft_get_fseries", "cufinufftf_get_fseries",
     "cufinufft_set_timing", "cufinufftf_set_timing",
     "cufinufft_get_timing", "cufinufftf_get_timing",
+    "cufinufft_execute_grad", "cufinufftf_execute_grad",
+    "cufinufft_group_make", "cufinufftf_group_make",
+    "cufinufft_group_execute", "cufinufftf_group_execute",
+    "cufinufft_group_setstream", "cufinufftf_group_setstream",
+    "cufinufft_group_destroy", "cufinufftf_group_destroy",
 ]

@@ -479,6 +479,130 @@ k_interp_gm(int M, SortedPts sp, cpx<T> *__restrict__ c, const cpx<T> *__restric
     }
 }
 
+// ES kernel and its derivative with respect to the point's grid coordinate
+// xr, at tap offset t = l - xr: phi(t) and d phi(l - xr)/d xr = -phi'(t) =
+// phi(t) beta c t / sqrt(1 - c t^2).  The derivative is integrable but
+// unbounded at |t| -> ns/2, where phi itself has fallen to exp(0) = 1, i.e.
+// exp(-beta) of its peak; the square root is floored to keep it finite.
+__device__ __forceinline__ void es_eval_grad(float t, const KerParams<float> &k, float &v,
+                                             float &dv)
+{
+    const float a = 1.0f - k.c * t * t;
+    const bool in = fabsf(t) < k.hw && a > 0.0f;
+    const float s = sqrtf(fmaxf(a, 1e-12f));
+    v = in ? expf(k.beta * s) : 0.0f;
+    dv = in ? v * k.beta * k.c * t / s : 0.0f;
+}
+__device__ __forceinline__ void es_eval_grad(double t, const KerParams<double> &k, double &v,
+                                             double &dv)
+{
+    const double a = 1.0 - k.c * t * t;
+    const bool in = fabs(t) < k.hw && a > 0.0;
+    const double s = sqrt(fmax(a, 1e-30));
+    v = in ? exp(k.beta * s) : 0.0;
+    dv = in ? v * k.beta * k.c * t / s : 0.0;
+}
+
+template <typename T, int NS>
+__device__ __forceinline__ int taps_global_grad(double v, const KerParams<T> &kp, T *ker, T *dker)
+{
+    const int start = (int)ceil(v - 0.5 * NS);
+    const T x1 = (T)((double)start - v);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) es_eval_grad(x1 + (T)i, kp, ker[i], dker[i]);
+    return start;
+}
+
+// Interp with gradient (type-2 plans, the differentiable form): one thread
+// per sorted point gathers the ns^DIM taps once and accumulates the value
+// c_j = sum_l fw_l prod_d phi_d and its DIM partial derivatives with respect
+// to the point's grid coordinates (one factor replaced by d phi / d xr_d),
+// scaled by d xr / d x = nf_d / (2 pi) into derivatives with respect to x_d.
+// dc holds DIM planes of ntr*M complex values.
+template <typename T, int DIM, int NS>
+__global__ void __launch_bounds__(256)
+k_interp_grad(int M, SortedPts sp, cpx<T> *__restrict__ c, cpx<T> *__restrict__ dc,
+              const cpx<T> *__restrict__ fw, BinGeom g, KerParams<T> kp, int ntr,
+              long long nftot, long long dc_plane)
+{
+    const int n1 = g.nf[0], n2 = g.nf[1], n3 = g.nf[2];
+    const T sc1 = (T)(n1 * kInv2Pi), sc2 = (T)(n2 * kInv2Pi), sc3 = (T)(n3 * kInv2Pi);
+    const JobRange jr = xcd_jobs((M + kPtChunk - 1) / kPtChunk);
+    for (long long ch = jr.begin; ch < jr.end; ch += jr.step) {
+        const int i = (int)(ch * kPtChunk) + (int)threadIdx.x;
+        if (i >= M) continue;
+        const int j = sp.idx[i];
+        T k1[NS], k2[NS], k3[NS], d1[NS], d2[NS], d3[NS];
+        const int s1 = taps_global_grad<T, NS>(sp.pos[i], kp, k1, d1);
+        int s2 = 0, s3 = 0;
+        if (DIM > 1) s2 = taps_global_grad<T, NS>(sp.pos[sp.stride + i], kp, k2, d2);
+        if (DIM > 2) s3 = taps_global_grad<T, NS>(sp.pos[2 * sp.stride + i], kp, k3, d3);
+        int ix[NS];
+#pragma unroll
+        for (int a = 0; a < NS; ++a) ix[a] = wrap1(s1 + a, n1);
+        for (int t = 0; t < ntr; ++t) {
+            const cpx<T> *f = fw + (long long)t * nftot;
+            // value, d/dx1, d/dx2, d/dx3 (re, im)
+            T vr = 0, vi = 0, gxr = 0, gxi = 0, gyr = 0, gyi = 0, gzr = 0, gzi = 0;
+#pragma unroll
+            for (int dz = 0; dz < (DIM > 2 ? NS : 1); ++dz) {
+                const int iz = DIM > 2 ? wrap1(s3 + dz, n3) : 0;
+                const T kz = DIM > 2 ? k3[dz] : (T)1;
+                const T dkz = DIM > 2 ? d3[dz] : (T)0;
+                T pr = 0, pi = 0, pxr = 0, pxi = 0, pyr = 0, pyi = 0;  // this z-plane
+#pragma unroll
+                for (int dy = 0; dy < (DIM > 1 ? NS : 1); ++dy) {
+                    const int iy = DIM > 1 ? wrap1(s2 + dy, n2) : 0;
+                    const T ky = DIM > 1 ? k2[dy] : (T)1;
+                    const T dky = DIM > 1 ? d2[dy] : (T)0;
+                    const cpx<T> *row = f + ((long long)iz * n2 + iy) * n1;
+                    T rr = 0, ri = 0, rxr = 0, rxi = 0;
+#pragma unroll
+                    for (int a = 0; a < NS; ++a) {
+                        const cpx<T> v = row[ix[a]];
+                        rr += v.x * k1[a];
+                        ri += v.y * k1[a];
+                        rxr += v.x * d1[a];
+                        rxi += v.y * d1[a];
+                    }
+                    pr += rr * ky;
+                    pi += ri * ky;
+                    pxr += rxr * ky;
+                    pxi += rxi * ky;
+                    pyr += rr * dky;
+                    pyi += ri * dky;
+                }
+                vr += pr * kz;
+                vi += pi * kz;
+                gxr += pxr * kz;
+                gxi += pxi * kz;
+                gyr += pyr * kz;
+                gyi += pyi * kz;
+                gzr += pr * dkz;
+                gzi += pi * dkz;
+            }
+            const long long o = (long long)t * M + j;
+            cpx<T> w;
+            w.x = vr;
+            w.y = vi;
+            c[o] = w;
+            w.x = gxr * sc1;
+            w.y = gxi * sc1;
+            dc[o] = w;
+            if (DIM > 1) {
+                w.x = gyr * sc2;
+                w.y = gyi * sc2;
+                dc[dc_plane + o] = w;
+            }
+            if (DIM > 2) {
+                w.x = gzr * sc3;
+                w.y = gzi * sc3;
+                dc[2 * dc_plane + o] = w;
+            }
+        }
+    }
+}
+
 // Tile of one subproblem: the bin's cells plus a halo of P = ceil(ns/2) on
 // each side; cell 0 of the tile is global index o[d] = bin*bs - P.
 template <int DIM, int NS>
@@ -543,6 +667,59 @@ __device__ __forceinline__ SubprobJob subprob_job(long long job, int total, cons
 // ds_add_f32 (measured), so the wider tile is 7.6x faster as well as more
 // accurate than FP32 accumulation.
 template <typename T, int DIM, int NS>
+__device__ __forceinline__ void spread_sm_job(const SubprobJob &J, int M, const SortedPts &sp,
+                                              const cpx<T> *__restrict__ c,
+                                              cpx<T> *__restrict__ fw, const BinGeom &g,
+                                              const KerParams<T> &kp, long long nftot,
+                                              unsigned char *smem)
+{
+    const Tile<DIM, NS> tl(g, J.b);
+    double *sre = reinterpret_cast<double *>(smem);
+    double *sim = sre + tl.n;
+    for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
+        sre[q] = 0.0;
+        sim[q] = 0.0;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < J.np; q += blockDim.x) {
+        const int i = J.p0 + q;
+        const int j = sp.idx[i];
+        T k1[NS], k2[NS], k3[NS];
+        const int l1 = taps_local<T, NS>(sp.pos[i], tl.o[0], tl.tw[0], kp, k1);
+        int l2 = 0, l3 = 0;
+        if (DIM > 1) l2 = taps_local<T, NS>(sp.pos[sp.stride + i], tl.o[1], tl.tw[1], kp, k2);
+        if (DIM > 2) l3 = taps_local<T, NS>(sp.pos[2 * sp.stride + i], tl.o[2], tl.tw[2], kp, k3);
+        const cpx<T> cv = c[(long long)J.t * M + j];
+#pragma unroll
+        for (int dz = 0; dz < (DIM > 2 ? NS : 1); ++dz) {
+            const T kz = DIM > 2 ? k3[dz] : (T)1;
+#pragma unroll
+            for (int dy = 0; dy < (DIM > 1 ? NS : 1); ++dy) {
+                const T kyz = (DIM > 1 ? k2[dy] : (T)1) * kz;
+                const T cr = cv.x * kyz, ci = cv.y * kyz;
+                const int row = ((l3 + dz) * tl.tw[1] + (l2 + dy)) * tl.tw[0] + l1;
+#pragma unroll
+                for (int a = 0; a < NS; ++a) {
+                    ladd(&sre[row + a], (double)(cr * k1[a]));
+                    ladd(&sim[row + a], (double)(ci * k1[a]));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    cpx<T> *f = fw + (long long)J.t * nftot;
+    for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
+        const T vr = (T)sre[q], vi = (T)sim[q];
+        if (vr != (T)0 || vi != (T)0) {
+            cpx<T> *p = f + tl.grid_offset(q, g);
+            gadd(&p->x, vr);
+            gadd(&p->y, vi);
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int DIM, int NS>
 __global__ void __launch_bounds__(256)
 k_spread_sm(int M, SortedPts sp, const cpx<T> *__restrict__ c, cpx<T> *__restrict__ fw,
             BinGeom g, KerParams<T> kp, SubprobArgs sa, int ntr, long long nftot)
@@ -550,59 +727,119 @@ k_spread_sm(int M, SortedPts sp, const cpx<T> *__restrict__ c, cpx<T> *__restric
     extern __shared__ __align__(16) unsigned char smem[];
     const int total = sa.counters[1];
     const JobRange jr = xcd_jobs((long long)total * ntr);
+    for (long long job = jr.begin; job < jr.end; job += jr.step)
+        spread_sm_job<T, DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g, kp, nftot, smem);
+}
+
+// One member of a plan group (many independent plans of one type, dimension,
+// precision and kernel width executed together): everything a subproblem job
+// of that plan needs, in device memory.
+template <typename T>
+struct GroupItem {
+    BinGeom g;
+    SortedPts sp;
+    SubprobArgs sa;
+    const cpx<T> *in;  // c (type 1) or fine grid (type 2)
+    cpx<T> *out;       // fine grid (type 1) or c (type 2)
+    long long nftot;
+    int M;
+    int ntr;
+};
+
+// Job offsets of a plan group: off[p] = sum over q < p of subproblems(q) *
+// ntr(q), computed on the device from each plan's setpts counters (so the
+// whole group launch sequence needs no host round trip and can be captured).
+template <typename T>
+__global__ void k_group_offsets(const GroupItem<T> *__restrict__ items, int n,
+                                long long *__restrict__ off)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        long long acc = 0;
+        for (int p = 0; p < n; ++p) {
+            off[p] = acc;
+            acc += (long long)items[p].sa.counters[1] * items[p].ntr;
+        }
+        off[n] = acc;
+    }
+}
+
+__device__ __forceinline__ int group_member(const long long *off, int n, long long job)
+{
+    int lo = 0, hi = n - 1;  // last p with off[p] <= job
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= job) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// SM spread of a whole plan group in one persistent launch over the global
+// job queue of all members.
+template <typename T, int DIM, int NS>
+__global__ void __launch_bounds__(256)
+k_spread_sm_group(const GroupItem<T> *__restrict__ items, int n,
+                  const long long *__restrict__ off, KerParams<T> kp)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const JobRange jr = xcd_jobs(off[n]);
     for (long long job = jr.begin; job < jr.end; job += jr.step) {
-        const SubprobJob J = subprob_job(job, total, sa);
-        const Tile<DIM, NS> tl(g, J.b);
-        double *sre = reinterpret_cast<double *>(smem);
-        double *sim = sre + tl.n;
-        for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
-            sre[q] = 0.0;
-            sim[q] = 0.0;
-        }
-        __syncthreads();
-        for (int q = threadIdx.x; q < J.np; q += blockDim.x) {
-            const int i = J.p0 + q;
-            const int j = sp.idx[i];
-            T k1[NS], k2[NS], k3[NS];
-            const int l1 = taps_local<T, NS>(sp.pos[i], tl.o[0], tl.tw[0], kp, k1);
-            int l2 = 0, l3 = 0;
-            if (DIM > 1) l2 = taps_local<T, NS>(sp.pos[sp.stride + i], tl.o[1], tl.tw[1], kp, k2);
-            if (DIM > 2)
-                l3 = taps_local<T, NS>(sp.pos[2 * sp.stride + i], tl.o[2], tl.tw[2], kp, k3);
-            const cpx<T> cv = c[(long long)J.t * M + j];
-#pragma unroll
-            for (int dz = 0; dz < (DIM > 2 ? NS : 1); ++dz) {
-                const T kz = DIM > 2 ? k3[dz] : (T)1;
-#pragma unroll
-                for (int dy = 0; dy < (DIM > 1 ? NS : 1); ++dy) {
-                    const T kyz = (DIM > 1 ? k2[dy] : (T)1) * kz;
-                    const T cr = cv.x * kyz, ci = cv.y * kyz;
-                    const int row = ((l3 + dz) * tl.tw[1] + (l2 + dy)) * tl.tw[0] + l1;
-#pragma unroll
-                    for (int a = 0; a < NS; ++a) {
-                        ladd(&sre[row + a], (double)(cr * k1[a]));
-                        ladd(&sim[row + a], (double)(ci * k1[a]));
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        cpx<T> *f = fw + (long long)J.t * nftot;
-        for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
-            const T vr = (T)sre[q], vi = (T)sim[q];
-            if (vr != (T)0 || vi != (T)0) {
-                cpx<T> *p = f + tl.grid_offset(q, g);
-                gadd(&p->x, vr);
-                gadd(&p->y, vi);
-            }
-        }
-        __syncthreads();
+        const int p = group_member(off, n, job);
+        const GroupItem<T> &it = items[p];
+        const SubprobJob J = subprob_job(job - off[p], it.sa.counters[1], it.sa);
+        spread_sm_job<T, DIM, NS>(J, it.M, it.sp, it.in, it.out, it.g, kp, it.nftot, smem);
     }
 }
 
 // SM interp (Interp_{2,3}d_Subprob, spreadinterp2d.cu:596-669): load the
 // padded tile of the subproblem's bin into LDS once (complex cells, so one
 // tap is one LDS read), then each thread gathers the taps of its points.
+template <typename T, int DIM, int NS>
+__device__ __forceinline__ void interp_sm_job(const SubprobJob &J, int M, const SortedPts &sp,
+                                              cpx<T> *__restrict__ c,
+                                              const cpx<T> *__restrict__ fw, const BinGeom &g,
+                                              const KerParams<T> &kp, long long nftot,
+                                              unsigned char *smem)
+{
+    const Tile<DIM, NS> tl(g, J.b);
+    cpx<T> *tile = reinterpret_cast<cpx<T> *>(smem);
+    const cpx<T> *f = fw + (long long)J.t * nftot;
+    for (int q = threadIdx.x; q < tl.n; q += blockDim.x) tile[q] = f[tl.grid_offset(q, g)];
+    __syncthreads();
+    for (int q = threadIdx.x; q < J.np; q += blockDim.x) {
+        const int i = J.p0 + q;
+        const int j = sp.idx[i];
+        T k1[NS], k2[NS], k3[NS];
+        const int l1 = taps_local<T, NS>(sp.pos[i], tl.o[0], tl.tw[0], kp, k1);
+        int l2 = 0, l3 = 0;
+        if (DIM > 1) l2 = taps_local<T, NS>(sp.pos[sp.stride + i], tl.o[1], tl.tw[1], kp, k2);
+        if (DIM > 2) l3 = taps_local<T, NS>(sp.pos[2 * sp.stride + i], tl.o[2], tl.tw[2], kp, k3);
+        T ar = 0, ai = 0;
+#pragma unroll
+        for (int dz = 0; dz < (DIM > 2 ? NS : 1); ++dz) {
+            const T kz = DIM > 2 ? k3[dz] : (T)1;
+#pragma unroll
+            for (int dy = 0; dy < (DIM > 1 ? NS : 1); ++dy) {
+                const T kyz = (DIM > 1 ? k2[dy] : (T)1) * kz;
+                const int row = ((l3 + dz) * tl.tw[1] + (l2 + dy)) * tl.tw[0] + l1;
+                T rr = 0, ri = 0;
+#pragma unroll
+                for (int a = 0; a < NS; ++a) {
+                    const cpx<T> v = tile[row + a];
+                    rr += v.x * k1[a];
+                    ri += v.y * k1[a];
+                }
+                ar += rr * kyz;
+                ai += ri * kyz;
+            }
+        }
+        cpx<T> o;
+        o.x = ar;
+        o.y = ai;
+        c[(long long)J.t * M + j] = o;
+    }
+    __syncthreads();
+}
+
 template <typename T, int DIM, int NS>
 __global__ void __launch_bounds__(256)
 k_interp_sm(int M, SortedPts sp, cpx<T> *__restrict__ c, const cpx<T> *__restrict__ fw,
@@ -611,47 +848,23 @@ k_interp_sm(int M, SortedPts sp, cpx<T> *__restrict__ c, const cpx<T> *__restric
     extern __shared__ __align__(16) unsigned char smem[];
     const int total = sa.counters[1];
     const JobRange jr = xcd_jobs((long long)total * ntr);
+    for (long long job = jr.begin; job < jr.end; job += jr.step)
+        interp_sm_job<T, DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g, kp, nftot, smem);
+}
+
+// SM interp of a whole plan group in one persistent launch.
+template <typename T, int DIM, int NS>
+__global__ void __launch_bounds__(256)
+k_interp_sm_group(const GroupItem<T> *__restrict__ items, int n,
+                  const long long *__restrict__ off, KerParams<T> kp)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const JobRange jr = xcd_jobs(off[n]);
     for (long long job = jr.begin; job < jr.end; job += jr.step) {
-        const SubprobJob J = subprob_job(job, total, sa);
-        const Tile<DIM, NS> tl(g, J.b);
-        cpx<T> *tile = reinterpret_cast<cpx<T> *>(smem);
-        const cpx<T> *f = fw + (long long)J.t * nftot;
-        for (int q = threadIdx.x; q < tl.n; q += blockDim.x) tile[q] = f[tl.grid_offset(q, g)];
-        __syncthreads();
-        for (int q = threadIdx.x; q < J.np; q += blockDim.x) {
-            const int i = J.p0 + q;
-            const int j = sp.idx[i];
-            T k1[NS], k2[NS], k3[NS];
-            const int l1 = taps_local<T, NS>(sp.pos[i], tl.o[0], tl.tw[0], kp, k1);
-            int l2 = 0, l3 = 0;
-            if (DIM > 1) l2 = taps_local<T, NS>(sp.pos[sp.stride + i], tl.o[1], tl.tw[1], kp, k2);
-            if (DIM > 2)
-                l3 = taps_local<T, NS>(sp.pos[2 * sp.stride + i], tl.o[2], tl.tw[2], kp, k3);
-            T ar = 0, ai = 0;
-#pragma unroll
-            for (int dz = 0; dz < (DIM > 2 ? NS : 1); ++dz) {
-                const T kz = DIM > 2 ? k3[dz] : (T)1;
-#pragma unroll
-                for (int dy = 0; dy < (DIM > 1 ? NS : 1); ++dy) {
-                    const T kyz = (DIM > 1 ? k2[dy] : (T)1) * kz;
-                    const int row = ((l3 + dz) * tl.tw[1] + (l2 + dy)) * tl.tw[0] + l1;
-                    T rr = 0, ri = 0;
-#pragma unroll
-                    for (int a = 0; a < NS; ++a) {
-                        const cpx<T> v = tile[row + a];
-                        rr += v.x * k1[a];
-                        ri += v.y * k1[a];
-                    }
-                    ar += rr * kyz;
-                    ai += ri * kyz;
-                }
-            }
-            cpx<T> o;
-            o.x = ar;
-            o.y = ai;
-            c[(long long)J.t * M + j] = o;
-        }
-        __syncthreads();
+        const int p = group_member(off, n, job);
+        const GroupItem<T> &it = items[p];
+        const SubprobJob J = subprob_job(job - off[p], it.sa.counters[1], it.sa);
+        interp_sm_job<T, DIM, NS>(J, it.M, it.sp, it.out, it.in, it.g, kp, it.nftot, smem);
     }
 }
 // ===========================================================================

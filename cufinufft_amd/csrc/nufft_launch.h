@@ -69,6 +69,29 @@ int run_amplify(int dim, const int *ms, const int *nf, cpx<T> *fw, const cpx<T> 
 template <typename T>
 int run_zero(cpx<T> *p, long long n, const Launch &L);
 
+// Interp of the value and its DIM coordinate derivatives (differentiable
+// form): dc holds DIM planes of dc_plane complex values.
+template <typename T>
+int run_interp_grad(int dim, int ns, int M, const SortBufs &sb, cpx<T> *c, cpx<T> *dc,
+                    const cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp, int ntr,
+                    long long nftot, long long dc_plane, const Launch &L);
+template <typename T, int DIM>
+int interp_grad_dim(int ns, int M, const SortBufs &sb, cpx<T> *c, cpx<T> *dc, const cpx<T> *fw,
+                    const BinGeom &g, const KerParams<T> &kp, int ntr, long long nftot,
+                    long long dc_plane, const Launch &L);
+
+// Plan groups: one persistent spread (type 1) or interp (type 2) launch over
+// the subproblems of every member.  items is a device array of n
+// GroupItem<T> (filled on the host with group_item_fill and copied), off a
+// device array of n+1 job offsets (written by the launch itself).
+size_t group_item_bytes(bool dbl);
+template <typename T, int DIM>
+int group_dim(int type, int ns, const void *items, int n, const long long *off,
+              const KerParams<T> &kp, const int *bs, long long jobs_bound, const Launch &L);
+template <typename T, int DIM>
+void group_item_fill(void *dst, const BinGeom &g, const SortBufs &sb, int maxsub, const void *in,
+                     void *out, long long nftot, int M, int ntr);
+
 // Per-(T, DIM) spread/interp launchers (one translation unit each).
 template <typename T, int DIM>
 int spread_dim(int method, int ns, int M, const SortBufs &sb, const cpx<T> *c, cpx<T> *fw,

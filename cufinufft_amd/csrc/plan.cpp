@@ -378,6 +378,7 @@ struct Plan {
                  int maxbatch_, const cufinufft_opts *o);
     int setpts(int M_, const T *x, const T *y, const T *z);
     int execute(cpx<T> *c, cpx<T> *fk);
+    int execute_grad(cpx<T> *c, cpx<T> *dc, cpx<T> *fk);
 };
 
 int default_opts(int type, int dim, cufinufft_opts *opts)
@@ -637,6 +638,44 @@ int Plan<T>::execute(cpx<T> *c, cpx<T> *fk)
     return 0;
 }
 
+// Type-2 execute that also returns the derivatives of every output with
+// respect to its point's coordinates: dc[d][t*M + j] = d c[t*M + j] / d x_d(j).
+template <typename T>
+int Plan<T>::execute_grad(cpx<T> *c, cpx<T> *dc, cpx<T> *fk)
+{
+    if (type != 2 || !pts_set) return 1;
+    if ((M > 0 && (!c || !dc)) || !fk) return 1;
+    DeviceGuard dg(opts.gpu_device_id);
+    KerParams<T> kp{sp.beta, sp.c, sp.hw};
+    const T *kers[3] = {fwker[0], fwker[1], fwker[2]};
+    const long long plane = (long long)M * ntransf;
+    int e = 0;
+    for (int b0 = 0; b0 < ntransf; b0 += maxbatch) {
+        const int blk = std::min(maxbatch, ntransf - b0);
+        cpx<T> *cb = c + (long long)b0 * M;
+        cpx<T> *db = dc + (long long)b0 * M;
+        cpx<T> *grid;
+        if (opts.gpu_spreadinterponly) {
+            grid = fk + (long long)b0 * nftot;
+        } else {
+            grid = fw;
+            tbegin();
+            e = run_amplify<T>(dim, ms, nf, fw, fk + (long long)b0 * ntot, kers, blk, L);
+            tend(kStageDeconv);
+            if (e) return e;
+            tbegin();
+            e = fft(fw, blk);
+            tend(kStageFft);
+            if (e) return e;
+        }
+        tbegin();
+        e = run_interp_grad<T>(dim, sp.ns, M, sb, cb, db, grid, geom, kp, blk, nftot, plane, L);
+        tend(kStageSpreadInterp);
+        if (e) return e;
+    }
+    return 0;
+}
+
 template <typename T>
 int get_timing(Plan<T> &p, double *ms, int *count)
 {
@@ -660,7 +699,182 @@ int get_timing(Plan<T> &p, double *ms, int *count)
 template struct Plan<float>;
 template struct Plan<double>;
 
+// ---------------------------------------------------------------------------
+// Plan group: many independent plans (same type, dimension, precision and
+// kernel width; any N, M, bins) executed as one launch sequence whose
+// spread / interp is a single persistent kernel over the subproblems of every
+// member.  The members' device arrays and the caller's c / fk pointers are
+// snapshotted into a device descriptor table at make time, so the group must
+// be re-made after any member's setpts.  The sequence runs on the group's own
+// stream, ordered after and before the caller's stream with events; with
+// use_graph it is captured into a hipGraph on the first call and replayed.
+template <typename T>
+struct Group {
+    std::vector<Plan<T> *> plans;
+    std::vector<cpx<T> *> c, fk;
+    int type = 0, dim = 0, ns = 0, dev = 0;
+    int bs[3] = {1, 1, 1};
+    long long jobs_bound = 0;
+    void *items = nullptr;
+    long long *off = nullptr;
+    hipStream_t stream = nullptr, user = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    Launch L;
+
+    ~Group()
+    {
+        DeviceGuard dg(dev);
+        if (gexec) hipGraphExecDestroy(gexec);
+        if (graph) hipGraphDestroy(graph);
+        if (ev_in) hipEventDestroy(ev_in);
+        if (ev_out) hipEventDestroy(ev_out);
+        if (stream) hipStreamDestroy(stream);
+        if (items) hipFree(items);
+        if (off) hipFree(off);
+    }
+
+    int make(int n, Plan<T> *const *ps, void *const *cs, void *const *fks)
+    {
+        if (n < 1 || !ps || !cs || !fks) return 1;
+        const Plan<T> &p0 = *ps[0];
+        type = p0.type;
+        dim = p0.dim;
+        ns = p0.sp.ns;
+        dev = p0.opts.gpu_device_id;
+        for (int i = 0; i < n; ++i) {
+            const Plan<T> *p = ps[i];
+            if (!p || !p->pts_set || p->type != type || p->dim != dim || p->sp.ns != ns ||
+                p->sp.beta != p0.sp.beta || p->method != 2 || p->maxbatch != p->ntransf ||
+                p->opts.gpu_spreadinterponly || p->opts.gpu_device_id != dev || !cs[i] ||
+                !fks[i])
+                return 1;
+            for (int d = 0; d < 3; ++d) bs[d] = std::max(bs[d], p->geom.bs[d]);
+            jobs_bound += p->sub_bound * p->ntransf;
+            plans.push_back(ps[i]);
+            c.push_back(static_cast<cpx<T> *>(cs[i]));
+            fk.push_back(static_cast<cpx<T> *>(fks[i]));
+        }
+        DeviceGuard dg(dev);
+        L.ncu = p0.L.ncu;
+        user = p0.L.stream;
+        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return kErrHip;
+        L.stream = stream;
+        if (hipEventCreateWithFlags(&ev_in, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ev_out, hipEventDisableTiming) != hipSuccess)
+            return kErrHip;
+        const size_t isz = group_item_bytes(sizeof(T) == 8);
+        std::vector<unsigned char> h(isz * n);
+        for (int i = 0; i < n; ++i) {
+            Plan<T> &p = *plans[i];
+            const void *in = type == 1 ? (const void *)c[i] : (const void *)p.fw;
+            void *out = type == 1 ? (void *)p.fw : (void *)c[i];
+            unsigned char *dst = h.data() + isz * i;
+            if (dim == 1)
+                group_item_fill<T, 1>(dst, p.geom, p.sb, p.opts.gpu_maxsubprobsize, in, out, p.nftot,
+                                      p.M, p.ntransf);
+            else if (dim == 2)
+                group_item_fill<T, 2>(dst, p.geom, p.sb, p.opts.gpu_maxsubprobsize, in, out, p.nftot,
+                                      p.M, p.ntransf);
+            else
+                group_item_fill<T, 3>(dst, p.geom, p.sb, p.opts.gpu_maxsubprobsize, in, out, p.nftot,
+                                      p.M, p.ntransf);
+        }
+        if (hipMalloc(&items, h.size()) != hipSuccess ||
+            hipMalloc((void **)&off, sizeof(long long) * (n + 1)) != hipSuccess)
+            return kErrHip;
+        if (hipMemcpy(items, h.data(), h.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return kErrHip;
+        return 0;
+    }
+
+    // The launch sequence, on the group stream.
+    int body()
+    {
+        int e = 0;
+        const Plan<T> &p0 = *plans[0];
+        KerParams<T> kp{p0.sp.beta, p0.sp.c, p0.sp.hw};
+        const int n = (int)plans.size();
+        std::vector<hipStream_t> saved(n);
+        for (int i = 0; i < n; ++i) {
+            saved[i] = plans[i]->L.stream;
+            plans[i]->L.stream = stream;  // the members' FFTs run on the group stream
+        }
+        auto spread_or_interp = [&]() {
+            if (dim == 1)
+                return group_dim<T, 1>(type, ns, items, n, off, kp, bs, jobs_bound, L);
+            if (dim == 2)
+                return group_dim<T, 2>(type, ns, items, n, off, kp, bs, jobs_bound, L);
+            return group_dim<T, 3>(type, ns, items, n, off, kp, bs, jobs_bound, L);
+        };
+        if (type == 1) {
+            for (int i = 0; i < n && !e; ++i)
+                e = run_zero<T>(plans[i]->fw, plans[i]->nftot * plans[i]->ntransf, L);
+            if (!e) e = spread_or_interp();
+            for (int i = 0; i < n && !e; ++i) {
+                Plan<T> &p = *plans[i];
+                const T *kers[3] = {p.fwker[0], p.fwker[1], p.fwker[2]};
+                e = p.fft(p.fw, p.ntransf);
+                if (!e) e = run_deconvolve<T>(dim, p.ms, p.nf, p.fw, fk[i], kers, p.ntransf, L);
+            }
+        } else {
+            for (int i = 0; i < n && !e; ++i) {
+                Plan<T> &p = *plans[i];
+                const T *kers[3] = {p.fwker[0], p.fwker[1], p.fwker[2]};
+                e = run_amplify<T>(dim, p.ms, p.nf, p.fw, fk[i], kers, p.ntransf, L);
+                if (!e) e = p.fft(p.fw, p.ntransf);
+            }
+            if (!e) e = spread_or_interp();
+        }
+        for (int i = 0; i < n; ++i) plans[i]->L.stream = saved[i];
+        return e;
+    }
+
+    int execute(int use_graph)
+    {
+        DeviceGuard dg(dev);
+        if (hipEventRecord(ev_in, user) != hipSuccess ||
+            hipStreamWaitEvent(stream, ev_in, 0) != hipSuccess)
+            return kErrHip;
+        int e = 0;
+        if (!use_graph) {
+            e = body();
+        } else {
+            if (!gexec) {
+                if (hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed) != hipSuccess)
+                    return kErrHip;
+                e = body();
+                hipGraph_t g = nullptr;
+                const hipError_t ce = hipStreamEndCapture(stream, &g);
+                if (e) {
+                    if (g) hipGraphDestroy(g);
+                    return e;
+                }
+                if (ce != hipSuccess || !g) return kErrHip;
+                graph = g;
+                if (hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
+                    gexec = nullptr;
+                    return kErrHip;
+                }
+            }
+            if (hipGraphLaunch(gexec, stream) != hipSuccess) return kErrHip;
+        }
+        if (e) return e;
+        if (hipEventRecord(ev_out, stream) != hipSuccess ||
+            hipStreamWaitEvent(user, ev_out, 0) != hipSuccess)
+            return kErrHip;
+        return 0;
+    }
+};
+
+template struct Group<float>;
+template struct Group<double>;
+
 }  // namespace cfa
+
+struct cufinufft_group_s : cfa::Group<double> {};
+struct cufinufftf_group_s : cfa::Group<float> {};
 
 struct cufinufft_plan_s : cfa::Plan<double> {};
 struct cufinufftf_plan_s : cfa::Plan<float> {};
@@ -781,6 +995,84 @@ int cufinufft_execute(void *c, void *fk, cufinufft_plan plan)
 int cufinufftf_execute(void *c, void *fk, cufinufftf_plan plan)
 {
     return plan ? plan->execute((cfa::cpx<float> *)c, (cfa::cpx<float> *)fk) : 1;
+}
+
+int cufinufft_group_make(int n, cufinufft_plan *plans, void **c, void **fk,
+                         cufinufft_group *out)
+{
+    if (!out || !plans || n < 1) return 1;
+    *out = nullptr;
+    std::vector<cfa::Plan<double> *> ps(plans, plans + n);
+    auto *g = new (std::nothrow) cufinufft_group_s();
+    if (!g) return 1;
+    const int e = g->make(n, ps.data(), c, fk);
+    if (e) {
+        delete g;
+        return e;
+    }
+    *out = g;
+    return 0;
+}
+int cufinufftf_group_make(int n, cufinufftf_plan *plans, void **c, void **fk,
+                          cufinufftf_group *out)
+{
+    if (!out || !plans || n < 1) return 1;
+    *out = nullptr;
+    std::vector<cfa::Plan<float> *> ps(plans, plans + n);
+    auto *g = new (std::nothrow) cufinufftf_group_s();
+    if (!g) return 1;
+    const int e = g->make(n, ps.data(), c, fk);
+    if (e) {
+        delete g;
+        return e;
+    }
+    *out = g;
+    return 0;
+}
+int cufinufft_group_execute(cufinufft_group g, int use_graph)
+{
+    return g ? g->execute(use_graph) : 1;
+}
+int cufinufftf_group_execute(cufinufftf_group g, int use_graph)
+{
+    return g ? g->execute(use_graph) : 1;
+}
+int cufinufft_group_setstream(cufinufft_group g, void *stream)
+{
+    if (!g) return 1;
+    g->user = (hipStream_t)stream;
+    return 0;
+}
+int cufinufftf_group_setstream(cufinufftf_group g, void *stream)
+{
+    if (!g) return 1;
+    g->user = (hipStream_t)stream;
+    return 0;
+}
+int cufinufft_group_destroy(cufinufft_group g)
+{
+    if (!g) return 1;
+    delete g;
+    return 0;
+}
+int cufinufftf_group_destroy(cufinufftf_group g)
+{
+    if (!g) return 1;
+    delete g;
+    return 0;
+}
+
+int cufinufft_execute_grad(void *c, void *dc, void *fk, cufinufft_plan plan)
+{
+    return plan ? plan->execute_grad((cfa::cpx<double> *)c, (cfa::cpx<double> *)dc,
+                                     (cfa::cpx<double> *)fk)
+                : 1;
+}
+int cufinufftf_execute_grad(void *c, void *dc, void *fk, cufinufftf_plan plan)
+{
+    return plan ? plan->execute_grad((cfa::cpx<float> *)c, (cfa::cpx<float> *)dc,
+                                     (cfa::cpx<float> *)fk)
+                : 1;
 }
 
 int cufinufft_destroy(cufinufft_plan plan)

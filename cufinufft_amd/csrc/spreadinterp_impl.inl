@@ -20,19 +20,28 @@ inline int grid_for_points(int M, int ncu)
 
 // Persistent grid for a subproblem kernel: as many workgroups as fit on the
 // device at the tile's LDS footprint, never more than there are jobs, rounded
-// to whole XCD groups.
+// to whole XCD groups.  The LDS attribute and occupancy are cached per kernel
+// and LDS size, so launches after the first make no runtime queries (which
+// keeps them legal inside a stream capture).
 template <typename K>
 int sm_grid(K kern, size_t lds, long long jobs, int ncu, int *grid)
 {
-    if (lds > 64 * 1024) {
+    static thread_local size_t attr_lds = 64 * 1024, occ_lds = 0;
+    static thread_local int occ_cached = 0;
+    if (lds > attr_lds) {
         if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds) != hipSuccess)
             return kErrLds;
+        attr_lds = lds;
     }
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 256, lds) != hipSuccess ||
-        occ < 1)
-        return kErrLds;
+    int occ = occ_cached;
+    if (lds != occ_lds || occ < 1) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 256, lds) != hipSuccess ||
+            occ < 1)
+            return kErrLds;
+        occ_cached = occ;
+        occ_lds = lds;
+    }
     long long g = (long long)occ * ncu;
     if (g > jobs) g = jobs;
     if (g >= kNumXcd) g -= g % kNumXcd;
@@ -84,6 +93,38 @@ int interp_ns(int method, int M, const SortBufs &sb, cpx<T> *c, const cpx<T> *fw
     return hipGetLastError() == hipSuccess ? 0 : kErrHip;
 }
 
+template <typename T, int DIM, int NS>
+int group_ns(int type, const GroupItem<T> *items, int n, const long long *off,
+             const KerParams<T> &kp, const int *bs, long long jobs_bound, const Launch &L)
+{
+    if (type == 1) {
+        const size_t lds = sm_tile_bytes(DIM, NS, bs, sizeof(double));
+        if (lds > kMaxLdsBytes) return kErrLds;
+        int grid = 0;
+        int e = sm_grid(k_spread_sm_group<T, DIM, NS>, lds, jobs_bound, L.ncu, &grid);
+        if (e) return e;
+        k_spread_sm_group<T, DIM, NS><<<grid, 256, lds, L.stream>>>(items, n, off, kp);
+    } else {
+        const size_t lds = sm_tile_bytes(DIM, NS, bs, sizeof(T));
+        if (lds > kMaxLdsBytes) return kErrLds;
+        int grid = 0;
+        int e = sm_grid(k_interp_sm_group<T, DIM, NS>, lds, jobs_bound, L.ncu, &grid);
+        if (e) return e;
+        k_interp_sm_group<T, DIM, NS><<<grid, 256, lds, L.stream>>>(items, n, off, kp);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : kErrHip;
+}
+
+template <typename T, int DIM, int NS>
+int interp_grad_ns(int M, const SortBufs &sb, cpx<T> *c, cpx<T> *dc, const cpx<T> *fw,
+                   const BinGeom &g, const KerParams<T> &kp, int ntr, long long nftot,
+                   long long dc_plane, const Launch &L)
+{
+    k_interp_grad<T, DIM, NS><<<grid_for_points(M, L.ncu), 256, 0, L.stream>>>(
+        M, sorted_pts(sb), c, dc, fw, g, kp, ntr, nftot, dc_plane);
+    return hipGetLastError() == hipSuccess ? 0 : kErrHip;
+}
+
 }  // namespace
 
 #define CFA_NS_CASES(F)                                                                      \
@@ -121,9 +162,66 @@ int interp_dim(int method, int ns, int M, const SortBufs &sb, cpx<T> *c, const c
     return 1;
 }
 
+template <typename T, int DIM>
+int interp_grad_dim(int ns, int M, const SortBufs &sb, cpx<T> *c, cpx<T> *dc, const cpx<T> *fw,
+                    const BinGeom &g, const KerParams<T> &kp, int ntr, long long nftot,
+                    long long dc_plane, const Launch &L)
+{
+    switch (ns) {
+#define CFA_CASE(N)                                                                          \
+    case N:                                                                                  \
+        return interp_grad_ns<T, DIM, N>(M, sb, c, dc, fw, g, kp, ntr, nftot, dc_plane, L);
+        CFA_NS_CASES(CFA_CASE)
+#undef CFA_CASE
+    }
+    return 1;
+}
+
+template <typename T, int DIM>
+int group_dim(int type, int ns, const void *items, int n, const long long *off,
+              const KerParams<T> &kp, const int *bs, long long jobs_bound, const Launch &L)
+{
+    const GroupItem<T> *it = static_cast<const GroupItem<T> *>(items);
+    k_group_offsets<T><<<1, 64, 0, L.stream>>>(it, n, const_cast<long long *>(off));
+    if (hipGetLastError() != hipSuccess) return kErrHip;
+    switch (ns) {
+#define CFA_CASE(N)                                                                          \
+    case N:                                                                                  \
+        return group_ns<T, DIM, N>(type, it, n, off, kp, bs, jobs_bound, L);
+        CFA_NS_CASES(CFA_CASE)
+#undef CFA_CASE
+    }
+    return 1;
+}
+
+template <typename T, int DIM>
+void group_item_fill(void *dst, const BinGeom &g, const SortBufs &sb, int maxsub, const void *in,
+                     void *out, long long nftot, int M, int ntr)
+{
+    GroupItem<T> &it = *static_cast<GroupItem<T> *>(dst);
+    it.g = g;
+    it.sp = SortedPts{sb.idx, sb.pos, sb.pos_stride};
+    it.sa = SubprobArgs{sb.s2b, sb.substart, sb.binstart, sb.binsize, sb.counters, maxsub};
+    it.in = static_cast<const cpx<T> *>(in);
+    it.out = static_cast<cpx<T> *>(out);
+    it.nftot = nftot;
+    it.M = M;
+    it.ntr = ntr;
+}
+
 }  // namespace cfa
 
 #define CFA_INSTANTIATE_SPREADINTERP(T, DIM)                                                  \
+    template int cfa::group_dim<T, DIM>(int, int, const void *, int, const long long *,      \
+                                        const cfa::KerParams<T> &, const int *, long long,   \
+                                        const cfa::Launch &);                                \
+    template void cfa::group_item_fill<T, DIM>(void *, const cfa::BinGeom &,                 \
+                                               const cfa::SortBufs &, int, const void *,     \
+                                               void *, long long, int, int);                 \
+    template int cfa::interp_grad_dim<T, DIM>(int, int, const cfa::SortBufs &, cfa::cpx<T> *, \
+                                              cfa::cpx<T> *, const cfa::cpx<T> *,            \
+                                              const cfa::BinGeom &, const cfa::KerParams<T> &, \
+                                              int, long long, long long, const cfa::Launch &); \
     template int cfa::spread_dim<T, DIM>(int, int, int, const cfa::SortBufs &,               \
                                          const cfa::cpx<T> *, cfa::cpx<T> *,                 \
                                          const cfa::BinGeom &, const cfa::KerParams<T> &,    \

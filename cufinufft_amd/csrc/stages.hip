@@ -16,6 +16,11 @@ inline int grid_1d(long long n, int ncu, int per_cu = 32)
 }
 }  // namespace
 
+size_t group_item_bytes(bool dbl)
+{
+    return dbl ? sizeof(GroupItem<double>) : sizeof(GroupItem<float>);
+}
+
 size_t sm_tile_bytes(int dim, int ns, const int *bs, size_t elem)
 {
     const int P = (ns + 1) / 2;
@@ -111,6 +116,19 @@ int run_interp(int method, int dim, int ns, int M, const SortBufs &sb, cpx<T> *c
 }
 
 template <typename T>
+int run_interp_grad(int dim, int ns, int M, const SortBufs &sb, cpx<T> *c, cpx<T> *dc,
+                    const cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp, int ntr,
+                    long long nftot, long long dc_plane, const Launch &L)
+{
+    if (M == 0) return 0;
+    if (dim == 1)
+        return interp_grad_dim<T, 1>(ns, M, sb, c, dc, fw, g, kp, ntr, nftot, dc_plane, L);
+    if (dim == 2)
+        return interp_grad_dim<T, 2>(ns, M, sb, c, dc, fw, g, kp, ntr, nftot, dc_plane, L);
+    return interp_grad_dim<T, 3>(ns, M, sb, c, dc, fw, g, kp, ntr, nftot, dc_plane, L);
+}
+
+template <typename T>
 int run_deconvolve(int dim, const int *ms, const int *nf, const cpx<T> *fw, cpx<T> *fk,
                    const T *const *ker, int ntr, const Launch &L)
 {
@@ -169,6 +187,9 @@ int run_zero(cpx<T> *p, long long n, const Launch &L)
     template int run_interp<T>(int, int, int, int, const SortBufs &, cpx<T> *,               \
                                const cpx<T> *, const BinGeom &, const KerParams<T> &, int,   \
                                long long, int, long long, const Launch &);                   \
+    template int run_interp_grad<T>(int, int, int, const SortBufs &, cpx<T> *, cpx<T> *,     \
+                                    const cpx<T> *, const BinGeom &, const KerParams<T> &,   \
+                                    int, long long, long long, const Launch &);              \
     template int run_deconvolve<T>(int, const int *, const int *, const cpx<T> *, cpx<T> *,  \
                                    const T *const *, int, const Launch &);                   \
     template int run_amplify<T>(int, const int *, const int *, cpx<T> *, const cpx<T> *,     \

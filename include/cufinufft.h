@@ -74,6 +74,40 @@ int cufinufftf_destroy(cufinufftf_plan plan);
 int cufinufft_setstream(cufinufft_plan plan, void *stream);
 int cufinufftf_setstream(cufinufftf_plan plan, void *stream);
 
+/* Plan groups: many independent plans of one type, dimension, precision and
+ * tolerance (any modes, points and bins; method 2; maxbatchsize = ntransf)
+ * executed together, with the spread (type 1) or interpolation (type 2) of
+ * every member in ONE persistent kernel launch over a global subproblem
+ * queue.  make snapshots the members' setpts arrays and the c / fk device
+ * pointers (plans[i] with c[i], fk[i]); re-make after any member's setpts.
+ * execute runs the group on its own stream, ordered after and before the
+ * caller's stream (setstream; null stream by default); use_graph = 1
+ * captures the launch sequence into a hipGraph on the first call and replays
+ * it afterwards.  For callers batching many small transforms (no counterpart
+ * in the reference). */
+typedef struct cufinufft_group_s *cufinufft_group;
+typedef struct cufinufftf_group_s *cufinufftf_group;
+int cufinufft_group_make(int n, cufinufft_plan *plans, void **c, void **fk,
+                         cufinufft_group *group);
+int cufinufftf_group_make(int n, cufinufftf_plan *plans, void **c, void **fk,
+                          cufinufftf_group *group);
+int cufinufft_group_execute(cufinufft_group group, int use_graph);
+int cufinufftf_group_execute(cufinufftf_group group, int use_graph);
+int cufinufft_group_setstream(cufinufft_group group, void *stream);
+int cufinufftf_group_setstream(cufinufftf_group group, void *stream);
+int cufinufft_group_destroy(cufinufft_group group);
+int cufinufftf_group_destroy(cufinufftf_group group);
+
+/* Type-2 execute that also returns each output's derivatives with respect to
+ * its point's coordinates (the differentiable form): c as in execute, and dc
+ * with dim planes of M*ntransf complex values, dc[d*M*ntransf + t*M + j] =
+ * d c[t*M + j] / d x_d[j] (x_0 = x, x_1 = y, x_2 = z).  One pass over the
+ * interpolation taps with the ES kernel and its derivative.  For autograd
+ * callers (gradients of losses with respect to point coordinates); returns 1
+ * for type-1 plans. */
+int cufinufft_execute_grad(void *c, void *dc, void *fk, cufinufft_plan plan);
+int cufinufftf_execute_grad(void *c, void *dc, void *fk, cufinufftf_plan plan);
+
 /* Plan facts, for tests and tuners: out[0..15] = ns, nf1, nf2, nf3,
  * numbins x, y, z, method, binsize x, y, z, maxsubprobsize, M,
  * total subproblems of the last setpts (synchronises), maxbatchsize,
