@@ -297,12 +297,139 @@ def cpu_baseline(w, pts, host_in, budget_pts):
 
 
 # ---------------------------------------------------------------- main
+def bench_group(a, dev):
+    """64 independent 2D type-1 plans (N per axis log-uniform in [64, 512],
+    M log-uniform in [1e3, 1e6], FP32, tol 1e-6): sequential plain plans vs
+    one plan group (one persistent spread launch) eager and hipGraph-replayed."""
+    import torch
+    from cufinufft_amd import cufinufft
+    from cufinufft_amd.cufinufft import cufinufft_group
+    rng = np.random.default_rng(4)
+    tol, n = 1e-6, 64
+    specs = []
+    for _ in range(n):
+        N = (int(round(np.exp(rng.uniform(np.log(64), np.log(512))))),
+             int(round(np.exp(rng.uniform(np.log(64), np.log(512))))))
+        M = int(round(np.exp(rng.uniform(np.log(1e3), np.log(1e6)))))
+        specs.append((N, M))
+    data = []
+    for (N, M) in specs:
+        x = torch.from_numpy(rng.uniform(-np.pi, np.pi, M).astype(np.float32)).to(dev)
+        y = torch.from_numpy(rng.uniform(-np.pi, np.pi, M).astype(np.float32)).to(dev)
+        c = torch.from_numpy(cplx_normal(rng, (M,), np.float32)).to(dev)
+        data.append((N, M, x, y, c))
+    totM = sum(s[1] for s in specs)
+
+    def make(opts):
+        plans, outs = [], []
+        for (N, M, x, y, c) in data:
+            p = cufinufft(1, N[::-1], eps=tol, dtype=np.float32, **opts)
+            p.set_pts(y, x)
+            plans.append(p)
+            outs.append(torch.empty(N[::-1], dtype=torch.complex64, device=dev))
+        return plans, outs
+
+    def timeit(fn, steps, warm):
+        for _ in range(warm):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    log("[bench] 64 plans, plain (reference defaults, one execute per plan)")
+    plans, outs = make({})
+    t_plain = timeit(lambda: [p.execute(d[4], o) for p, d, o in zip(plans, data, outs)],
+                     a.steps, a.warmup)
+    del plans, outs
+    opts = dict(gpu_method=2, gpu_binsizex=32, gpu_binsizey=32, gpu_maxsubprobsize=1024)
+    plans, outs = make(opts)
+    grp = cufinufft_group(plans, [d[4] for d in data], outs)
+    t_eager = timeit(lambda: grp.execute(False), a.steps, a.warmup)
+    t_graph = timeit(lambda: grp.execute(True), a.steps, a.warmup)
+    log(f"[bench] plain {t_plain * 1e3:.3f} ms, group eager {t_eager * 1e3:.3f} ms, "
+        f"group graph {t_graph * 1e3:.3f} ms for {totM} points")
+    # accuracy of two members against the exact sum
+    errs = []
+    for k in (0, n // 2):
+        (N, M, x, y, c) = data[k]
+        kidx = np.random.default_rng(9 + k).choice(N[0] * N[1], 500, replace=False)
+        ref = direct_type1([x.cpu().numpy(), y.cpu().numpy()], c.cpu().numpy(), N, +1, kidx)
+        errs.append(rel_l2(outs[k].cpu().numpy().ravel()[kidx], ref))
+    t = min(t_eager, t_graph)
+    return {
+        "metric": "NU pts/s per execute at rel-l2 <= tol: tuned/batched vs plain, 1 MI355X",
+        "value": totM / t, "unit": "NU pts/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": t * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "64 independent 2D type-1 plans, N/axis log-uniform [64,512], "
+                               "M log-uniform [1e3,1e6], FP32 tol 1e-6, one plan group",
+                   "total_points": totM, "group_opts": opts},
+        "rel_l2": {"members": errs, "tol": tol},
+        "plain": {"value": totM / t_plain, "ms_per_step": t_plain * 1e3},
+        "group_eager_ms": t_eager * 1e3, "group_graph_ms": t_graph * 1e3,
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
+def bench_autograd(a, dev):
+    """Differentiable form: 2D type-2 FP32 tol 1e-6, N=320^2, 320 golden-angle
+    radial spokes x 640 samples (jitter 1e-3); one step = forward + backward
+    of ||c - target||^2 to modes and point coordinates."""
+    import torch
+    from cufinufft_amd.torch_nufft import nufft_type2
+    rng = np.random.default_rng(5)
+    N, nspk, nsamp = (320, 320), 320, 640
+    ang = np.mod(np.arange(nspk) * np.pi / ((1 + 5 ** 0.5) / 2), np.pi)
+    rad = -np.pi + 2 * np.pi * np.arange(nsamp) / nsamp
+    x = np.outer(np.cos(ang), rad).ravel() + 1e-3 * rng.standard_normal(nspk * nsamp)
+    y = np.outer(np.sin(ang), rad).ravel() + 1e-3 * rng.standard_normal(nspk * nsamp)
+    P = np.mod(np.stack([x, y]) + np.pi, 2 * np.pi) - np.pi
+    M = P.shape[1]
+    pts = torch.tensor(P.astype(np.float32), device=dev, requires_grad=True)
+    fk = torch.tensor(cplx_normal(rng, N[::-1], np.float32), device=dev, requires_grad=True)
+    target = torch.tensor(cplx_normal(rng, (M,), np.float32), device=dev)
+
+    def step():
+        pts.grad = None
+        fk.grad = None
+        c = nufft_type2(pts, fk, isign=-1, eps=1e-6)
+        loss = (c - target).abs().pow(2).sum()
+        loss.backward()
+        return c
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        c = step()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / a.steps
+    pidx = rng.choice(M, 1000, replace=False)
+    ref = direct_type2([P[0], P[1]], fk.detach().cpu().numpy(), -1, pidx)
+    err = rel_l2(c.detach().cpu().numpy()[pidx], ref)
+    return {
+        "metric": "NU pts/s per forward+backward step (differentiable type 2), 1 MI355X",
+        "value": M / t, "unit": "NU pts/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": t * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "2D type-2 FP32 tol 1e-6, N=320^2, 320 radial spokes x 640 "
+                               "samples, forward + backward to modes and coordinates"},
+        "rel_l2": {"forward": err, "tol": 1e-6},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c2",
+                    choices=sorted(WORKLOADS) + ["c4many", "c5"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
@@ -310,6 +437,9 @@ def main():
                     help="JSON dict of cufinufft_opts fields for the tuned plan (skips the sweep)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration)")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="with --opts: time only that plan (no plain plan, no accuracy "
+                         "check, no CPU baseline) -- for profiler runs")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
@@ -327,6 +457,10 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] copied {a.copy_calib} bytes x3")
         return 0
+    if a.workload in ("c4many", "c5"):
+        res = (bench_group if a.workload == "c4many" else bench_autograd)(a, dev)
+        print(json.dumps(res), flush=True)
+        return 0
     w = WORKLOADS[a.workload]
     dt = np.dtype(w["dtype"])
     rng = np.random.default_rng(w["seed"])
@@ -342,6 +476,13 @@ def main():
     torch.cuda.synchronize()
 
     maxbatch = min(T, 8)
+    if a.kernel_only:
+        if not a.opts:
+            log("[bench] --kernel-only needs --opts")
+            return 2
+        r = run_plan(w, dev, dpts, din, json.loads(a.opts), a.steps, a.warmup, maxbatch)
+        log(f"[bench] kernel-only: {r['t_exec'] * 1e3:.3f} ms/exec, stages {r['stages']}")
+        return 0
     log("[bench] plain plan (reference default options)")
     plain = run_plan(w, dev, dpts, din, {}, a.steps, a.warmup, maxbatch)
     log("[bench] exact FP64 reference at 1000 sampled outputs")

@@ -4,7 +4,7 @@ Mirrors python/cufinufft/_cufinufft.py of cuFINUFFT v1.3: the same NufftOpts
 structure, the same function objects (_default_opts, _make_plan[f],
 _set_pts[f], _exec_plan[f], _destroy_plan[f]) with the same argtypes, plus the
 library's extension entry points (_set_stream[f], _get_info[f],
-_get_sort[f], _get_fseries[f]).
+_get_sort[f], _get_fseries[f], _get_kernel_poly[f]).
 
 Loading fails loudly: there is no CPU fallback.
 """
@@ -138,6 +138,13 @@ _get_fseriesf = lib.cufinufftf_get_fseries
 _get_fseriesf.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
 _get_fseriesf.restype = c_int
 
+_get_kernel_poly = lib.cufinufft_get_kernel_poly
+_get_kernel_poly.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
+_get_kernel_poly.restype = c_int
+_get_kernel_polyf = lib.cufinufftf_get_kernel_poly
+_get_kernel_polyf.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p]
+_get_kernel_polyf.restype = c_int
+
 _set_timing = lib.cufinufft_set_timing
 _set_timing.argtypes = [c_void_p, c_int]
 _set_timing.restype = c_int
@@ -184,6 +191,7 @@ EXPORTED = [
     "cufinufft_get_info", "cufinufftf_get_info",
     "cufinufft_get_sort", "cufinufftf_get_sort",
     "cufinufft_get_fseries", "cufinufftf_get_fseries",
+    "cufinufft_get_kernel_poly", "cufinufftf_get_kernel_poly",
     "cufinufft_set_timing", "cufinufftf_set_timing",
     "cufinufft_get_timing", "cufinufftf_get_timing",
     "cufinufft_execute_grad", "cufinufftf_execute_grad",

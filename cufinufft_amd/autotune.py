@@ -13,9 +13,10 @@ if ``CUFINUFFT_AMD_TUNE_CACHE`` names a JSON file, across processes.
     plan = cufinufft(1, (256, 256, 256), eps=1e-5, dtype=np.float32, **best)
 
 The candidate list follows the hardware: spread tiles accumulate in FP64 LDS
-(16 bytes per cell) and interp tiles hold one complex per cell, both capped
-at the 160 KiB LDS of a gfx950 workgroup; plans whose tile does not fit are
-skipped (makeplan reports it).
+(16 bytes per cell) for FP64 plans and in u32 fixed point (8 bytes per cell)
+for FP32 plans, interp tiles hold one complex per cell, all capped at the
+160 KiB LDS of a gfx950 workgroup; plans whose tile does not fit are skipped
+(makeplan reports it).
 """
 import json
 import os
@@ -35,6 +36,10 @@ def candidates(nufft_type, dim):
     """Option sets to try (cufinufft_opts fields)."""
     if dim == 3:
         bins = [(16, 16, 2), (8, 8, 8), (8, 8, 4), (16, 8, 4), (4, 4, 4), (16, 16, 4)]
+        if nufft_type == 1:
+            # larger tiles: less halo per interior cell in the flush; they fit
+            # the LDS only with the 8-byte fixed-point cells of FP32 plans
+            bins += [(16, 16, 8), (32, 16, 8), (16, 16, 16)]
     elif dim == 2:
         bins = [(32, 32, 1), (16, 16, 1), (64, 32, 1), (64, 64, 1), (32, 16, 1)]
     else:
@@ -45,6 +50,11 @@ def candidates(nufft_type, dim):
         for msub in subs:
             out.append(dict(gpu_method=2, gpu_binsizex=b[0], gpu_binsizey=b[1],
                             gpu_binsizez=b[2], gpu_maxsubprobsize=msub))
+            # cell-ordered points inside bins (gpu_sort=2): fewer LDS bank
+            # conflicts for the interpolation gathers
+            if nufft_type == 2:
+                out.append(dict(gpu_method=2, gpu_sort=2, gpu_binsizex=b[0], gpu_binsizey=b[1],
+                                gpu_binsizez=b[2], gpu_maxsubprobsize=msub))
     # GM-sort: the bin size only sets the sort order
     for b in bins[:2]:
         out.append(dict(gpu_method=1, gpu_binsizex=b[0], gpu_binsizey=b[1], gpu_binsizez=b[2],
@@ -105,10 +115,14 @@ def tune(nufft_type, modes, pts, eps, dtype=np.float32, n_trans=1, maxbatchsize=
         args = (out, inp)
     table = []
     best, best_t = None, float("inf")
-    for opts in (cands or candidates(nufft_type, dim)):
+
+    def trial(opts):
+        kw = dict(upsampfac=upsampfac)
+        kw.update(opts)
+        plan = None
         try:
             plan = cufinufft(nufft_type, shape, n_trans=n_trans, eps=eps, dtype=dtype,
-                             maxbatchsize=maxbatchsize, upsampfac=upsampfac, **opts)
+                             maxbatchsize=maxbatchsize, **kw)
             plan.set_pts(*pts)
             plan.execute(*args)
             torch.cuda.synchronize()
@@ -120,16 +134,31 @@ def tune(nufft_type, modes, pts, eps, dtype=np.float32, n_trans=1, maxbatchsize=
         except RuntimeError as e:
             if log:
                 log(f"  skip {opts}: {e}")
-            continue
+            return None
         finally:
             plan = None
         table.append((opts, t))
         if log:
             log(f"  {opts}: {t:.3f} ms")
-        if t < best_t:
+        return t
+
+    for opts in (cands or candidates(nufft_type, dim)):
+        t = trial(opts)
+        if t is not None and t < best_t:
             best, best_t = opts, t
     if best is None:
         raise RuntimeError("no candidate option set could be planned")
+    if cands is None:
+        # second stage on the winner: the polynomial kernel (sigma = 2 only)
+        # and the smaller sigma = 1.25 grid (wider kernel, smaller FFT)
+        extra = []
+        if upsampfac == 2.0:
+            extra.append(dict(best, gpu_kerevalmeth=1))
+            extra.append(dict(best, upsampfac=1.25))
+        for opts in extra:
+            t = trial(opts)
+            if t is not None and t < best_t:
+                best, best_t = opts, t
     _CACHE[key] = best
     if use_cache:
         _save_disk()

@@ -29,6 +29,12 @@ struct KerParams {
     T beta;  // ES beta
     T c;     // ES c = 4/ns^2
     T hw;    // half width ns/2
+    // Piecewise-polynomial kernel (gpu_kerevalmeth = 1): hdeg > 0 selects it.
+    // hc[k*ns + i] is the z^k coefficient of tap i, z = 2(start - v) + ns - 1
+    // in [-1, 1); a second table of the same size follows at hc + (hdeg+1)*ns
+    // holding the kernel divided by e^beta (peak 1).
+    const T *hc;
+    int hdeg;
 };
 
 // Points as the execute kernels read them, in bin-sorted order: idx[i] is
@@ -137,10 +143,27 @@ __device__ __forceinline__ double tap_pos(double v, int n)
 
 // First tap index and the ns kernel values of one axis:
 // start = ceil(v - ns/2) (exact in double), ker[i] = phi(start + i - v).
+// Horner evaluation of the ns per-tap polynomials at z (all taps of one
+// point share z); coefficient addresses are lane-uniform.
+template <typename T, int NS>
+__device__ __forceinline__ void horner_taps(T z, const T *__restrict__ hc, int deg, T *ker)
+{
+#pragma unroll
+    for (int i = 0; i < NS; ++i) ker[i] = hc[deg * NS + i];
+    for (int k = deg - 1; k >= 0; --k) {
+#pragma unroll
+        for (int i = 0; i < NS; ++i) ker[i] = ker[i] * z + hc[k * NS + i];
+    }
+}
+
 template <typename T, int NS>
 __device__ __forceinline__ int taps_global(double v, const KerParams<T> &kp, T *ker)
 {
     const int start = (int)ceil(v - 0.5 * NS);
+    if (kp.hdeg > 0) {
+        horner_taps<T, NS>((T)(2.0 * ((double)start - v) + (NS - 1)), kp.hc, kp.hdeg, ker);
+        return start;
+    }
     const T x1 = (T)((double)start - v);
 #pragma unroll
     for (int i = 0; i < NS; ++i) ker[i] = es_eval(x1 + (T)i, kp);
@@ -156,8 +179,12 @@ template <typename T, int NS>
 __device__ __forceinline__ int taps_local(double v, int o, int tw, const KerParams<T> &kp,
                                           T *ker)
 {
-    int l = (int)ceil(v - 0.5 * NS) - o;
-    l = l < 0 ? 0 : (l > tw - NS ? tw - NS : l);
+    const int l0 = (int)ceil(v - 0.5 * NS) - o;
+    const int l = l0 < 0 ? 0 : (l0 > tw - NS ? tw - NS : l0);
+    if (kp.hdeg > 0 && l == l0) {
+        horner_taps<T, NS>((T)(2.0 * ((double)(o + l) - v) + (NS - 1)), kp.hc, kp.hdeg, ker);
+        return l;
+    }
     const T x1 = (T)((double)(o + l) - v);
 #pragma unroll
     for (int i = 0; i < NS; ++i) ker[i] = es_eval(x1 + (T)i, kp);
@@ -206,6 +233,55 @@ k_bin_count(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__
         const int b = point_bin<T, DIM>(x, y, z, i, g);
         binof[i] = b;
         rank[i] = atomicAdd(&binsize[b], 1);
+    }
+}
+
+// Fine sort key: the bin (as above) then the fine-grid cell of the point
+// inside its bin, key = bin * vol + local cell (vol = cells per bin).  Sorting
+// by this key orders each bin's points by cell, so neighbouring lanes of a
+// wave touch neighbouring LDS cells (or the same cell, which the LDS
+// broadcasts); the per-bin counts and starts are unchanged.
+template <typename T>
+__device__ __forceinline__ void bin_and_cell(double v, int bs, int nb, int &b, int &lc)
+{
+    const T xt = (T)v;
+    const T q = floor(xt / (T)bs);
+    b = q >= (T)0 ? (q < (T)nb ? (int)q : nb - 1) : 0;
+    const T f = floor(xt);
+    int c = f >= (T)0 ? (f < (T)2147483647 ? (int)f : 2147483647) : 0;
+    c -= b * bs;
+    lc = c < 0 ? 0 : (c >= bs ? bs - 1 : c);
+}
+
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+k_key_count(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
+            BinGeom g, int *__restrict__ keycount, int *__restrict__ rank,
+            int *__restrict__ keyof)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+        int b0, c0, b1 = 0, c1 = 0, b2 = 0, c2 = 0;
+        bin_and_cell<T>(rescale_exact(x[i], g.nf[0]), g.bs[0], g.nb[0], b0, c0);
+        if (DIM > 1) bin_and_cell<T>(rescale_exact(y[i], g.nf[1]), g.bs[1], g.nb[1], b1, c1);
+        if (DIM > 2) bin_and_cell<T>(rescale_exact(z[i], g.nf[2]), g.bs[2], g.nb[2], b2, c2);
+        const int bin = b0 + g.nb[0] * (b1 + g.nb[1] * b2);
+        const int vol = g.bs[0] * g.bs[1] * g.bs[2];
+        const int key = bin * vol + c0 + g.bs[0] * (c1 + g.bs[1] * c2);
+        keyof[i] = key;
+        rank[i] = atomicAdd(&keycount[key], 1);
+    }
+}
+
+// Per-bin size and start from the key starts: the bin's keys are contiguous.
+static __global__ void __launch_bounds__(256)
+k_bins_from_keys(long long nbins, int vol, int M, const int *__restrict__ keystart,
+                 int *__restrict__ binsize)
+{
+    for (long long b = blockIdx.x * (long long)blockDim.x + threadIdx.x; b < nbins;
+         b += (long long)gridDim.x * blockDim.x) {
+        const int s0 = keystart[b * vol];
+        const int s1 = b + 1 < nbins ? keystart[(b + 1) * vol] : M;
+        binsize[b] = s1 - s0;
     }
 }
 
@@ -322,7 +398,7 @@ k_scan_parts(int2 *__restrict__ part, int nparts, int *__restrict__ counters,
         carry = add2(carry, tot);
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && counters) {
         counters[0] = carry.x;
         counters[1] = carry.y;
         *substart_end = carry.y;
@@ -348,7 +424,7 @@ k_scan_apply(const int *__restrict__ cnt, long long n, int maxsub, const int2 *_
     for (int k = 0; k < kScanItems; ++k) {
         if (base + k < n) {
             binstart[base + k] = run.x;
-            substart[base + k] = run.y;
+            if (substart) substart[base + k] = run.y;
         }
         run = add2(run, v[k]);
     }
@@ -666,6 +742,125 @@ __device__ __forceinline__ SubprobJob subprob_job(long long job, int total, cons
 // ds_add_f64 sustains about 2.5 lane-ops per CU clock against 0.33 for
 // ds_add_f32 (measured), so the wider tile is 7.6x faster as well as more
 // accurate than FP32 accumulation.
+// Kernel values normalised by their peak: phi(t) / exp(beta) in (0, 1].
+template <int NS>
+__device__ __forceinline__ int taps_local_unit(double v, int o, int tw, const KerParams<float> &kp,
+                                               float *ker)
+{
+    const int l0 = (int)ceil(v - 0.5 * NS) - o;
+    const int l = l0 < 0 ? 0 : (l0 > tw - NS ? tw - NS : l0);
+    if (kp.hdeg > 0 && l == l0) {
+        horner_taps<float, NS>((float)(2.0 * ((double)(o + l) - v) + (NS - 1)),
+                               kp.hc + (kp.hdeg + 1) * NS, kp.hdeg, ker);
+        return l;
+    }
+    const float x1 = (float)((double)(o + l) - v);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        const float t = x1 + (float)i;
+        const float a = fmaxf(1.0f - kp.c * t * t, 0.0f);
+        ker[i] = fabsf(t) < kp.hw ? expf(kp.beta * (sqrtf(a) - 1.0f)) : 0.0f;
+    }
+    return l;
+}
+
+// Sum of a float over the 256 threads of the workgroup (returned to all).
+__device__ __forceinline__ float block_sum256(float v, float *scratch)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) scratch[wid] = v;
+    __syncthreads();
+    const float s = scratch[0] + scratch[1] + scratch[2] + scratch[3];
+    __syncthreads();
+    return s;
+}
+
+// LDS bytes of the single-precision fixed-point spread: 32-bit re / im per
+// tile cell, the job's strengths staged as complex floats, and 4 floats of
+// reduction scratch.
+__host__ __device__ inline size_t fx_spread_lds(size_t cells, int maxsub)
+{
+    return cells * 8 + (size_t)maxsub * 8 + 16;
+}
+
+// Single-precision SM spread job with 32-bit fixed-point LDS accumulation.
+// On gfx950 ds_add_u32 sustains 8-14 lane-ops per CU clock against 2.5 for
+// ds_add_f64 and 0.33 for ds_add_f32 (measured).  Exactness by construction:
+// with the kernel normalised to phi/exp(beta) <= 1 per axis, no tile cell can
+// exceed sum_p (|re c_p| + |im c_p|) =: A over the job's points, so the job
+// scales contributions by the power of two S = 2^30 / A (rounded down), well
+// inside int32; each contribution is rounded to nearest, an absolute error of
+// at most A * 2^-31 per tap, below FP32 rounding of the same sums.  The flush
+// multiplies by exp(DIM beta) / S.
+template <int DIM, int NS>
+__device__ __forceinline__ void spread_sm_job_fx(const SubprobJob &J, int M, const SortedPts &sp,
+                                                 const cpx<float> *__restrict__ c,
+                                                 cpx<float> *__restrict__ fw, const BinGeom &g,
+                                                 const KerParams<float> &kp, long long nftot,
+                                                 unsigned char *smem, int maxsub)
+{
+    const Tile<DIM, NS> tl(g, J.b);
+    int *sre = reinterpret_cast<int *>(smem);
+    int *sim = sre + tl.n;
+    cpx<float> *cst = reinterpret_cast<cpx<float> *>(sim + tl.n);  // 8*tl.n bytes in: aligned
+    float *red = reinterpret_cast<float *>(cst + maxsub);
+    for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
+        sre[q] = 0;
+        sim[q] = 0;
+    }
+    float part = 0.0f;
+    for (int q = threadIdx.x; q < J.np; q += blockDim.x) {
+        const cpx<float> cv = c[(long long)J.t * M + sp.idx[J.p0 + q]];
+        cst[q] = cv;
+        part += fabsf(cv.x) + fabsf(cv.y);
+    }
+    const float A = block_sum256(part, red);  // also orders the stores above
+    int ex = 0;
+    frexpf(A, &ex);                            // A < 2^ex
+    const int sh = A > 0.0f ? 30 - ex : 0;     // S = 2^sh, applied with ldexp (no overflow)
+    for (int q = threadIdx.x; q < J.np; q += blockDim.x) {
+        const int i = J.p0 + q;
+        float k1[NS], k2[NS], k3[NS];
+        const int l1 = taps_local_unit<NS>(sp.pos[i], tl.o[0], tl.tw[0], kp, k1);
+        int l2 = 0, l3 = 0;
+        if (DIM > 1) l2 = taps_local_unit<NS>(sp.pos[sp.stride + i], tl.o[1], tl.tw[1], kp, k2);
+        if (DIM > 2) l3 = taps_local_unit<NS>(sp.pos[2 * sp.stride + i], tl.o[2], tl.tw[2], kp, k3);
+        const cpx<float> cv = cst[q];
+        const float sr = ldexpf(cv.x, sh), si = ldexpf(cv.y, sh);
+#pragma unroll
+        for (int dz = 0; dz < (DIM > 2 ? NS : 1); ++dz) {
+            const float kz = DIM > 2 ? k3[dz] : 1.0f;
+#pragma unroll
+            for (int dy = 0; dy < (DIM > 1 ? NS : 1); ++dy) {
+                const float kyz = (DIM > 1 ? k2[dy] : 1.0f) * kz;
+                const float cr = sr * kyz, ci = si * kyz;
+                const int row = ((l3 + dz) * tl.tw[1] + (l2 + dy)) * tl.tw[0] + l1;
+#pragma unroll
+                for (int a = 0; a < NS; ++a) {
+                    atomicAdd(&sre[row + a], __float2int_rn(cr * k1[a]));
+                    atomicAdd(&sim[row + a], __float2int_rn(ci * k1[a]));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // e^(DIM*beta) in FP64: in FP32 the rounding of DIM*beta (~50) alone
+    // would scale the whole tile by up to 2e-6
+    const float back = (float)ldexp(exp((double)DIM * (double)kp.beta), -sh);
+    cpx<float> *f = fw + (long long)J.t * nftot;
+    for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
+        const int ir = sre[q], ii = sim[q];
+        if (ir != 0 || ii != 0) {
+            cpx<float> *p = f + tl.grid_offset(q, g);
+            gadd(&p->x, (float)ir * back);
+            gadd(&p->y, (float)ii * back);
+        }
+    }
+    __syncthreads();
+}
+
 template <typename T, int DIM, int NS>
 __device__ __forceinline__ void spread_sm_job(const SubprobJob &J, int M, const SortedPts &sp,
                                               const cpx<T> *__restrict__ c,
@@ -727,8 +922,14 @@ k_spread_sm(int M, SortedPts sp, const cpx<T> *__restrict__ c, cpx<T> *__restric
     extern __shared__ __align__(16) unsigned char smem[];
     const int total = sa.counters[1];
     const JobRange jr = xcd_jobs((long long)total * ntr);
-    for (long long job = jr.begin; job < jr.end; job += jr.step)
-        spread_sm_job<T, DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g, kp, nftot, smem);
+    for (long long job = jr.begin; job < jr.end; job += jr.step) {
+        if constexpr (sizeof(T) == 4)
+            spread_sm_job_fx<DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g, kp, nftot,
+                                      smem, sa.maxsub);
+        else
+            spread_sm_job<T, DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g, kp, nftot,
+                                      smem);
+    }
 }
 
 // One member of a plan group (many independent plans of one type, dimension,
@@ -786,7 +987,11 @@ k_spread_sm_group(const GroupItem<T> *__restrict__ items, int n,
         const int p = group_member(off, n, job);
         const GroupItem<T> &it = items[p];
         const SubprobJob J = subprob_job(job - off[p], it.sa.counters[1], it.sa);
-        spread_sm_job<T, DIM, NS>(J, it.M, it.sp, it.in, it.out, it.g, kp, it.nftot, smem);
+        if constexpr (sizeof(T) == 4)
+            spread_sm_job_fx<DIM, NS>(J, it.M, it.sp, it.in, it.out, it.g, kp, it.nftot, smem,
+                                      it.sa.maxsub);
+        else
+            spread_sm_job<T, DIM, NS>(J, it.M, it.sp, it.in, it.out, it.g, kp, it.nftot, smem);
     }
 }
 

@@ -29,6 +29,10 @@ struct SortBufs {
     long long pos_stride = 0;   // capacity in points of each pos plane
     int *counters = nullptr;    // [2]: points, subproblems
     void *scan_part = nullptr;  // int2[ceil(nbins / chunk)]
+    // fine sort (key = bin * cells-per-bin + cell in bin), when enabled
+    int *kcount = nullptr;       // [nkeys]
+    int *kstart = nullptr;       // [nkeys]
+    void *kscan_part = nullptr;  // int2[ceil(nkeys / chunk)]
 };
 
 struct Launch {
@@ -41,6 +45,10 @@ struct Launch {
 template <typename T>
 int run_bin_sort(int dim, const BinGeom &g, long long nbins, int M, const T *x, const T *y,
                  const T *z, int maxsub, bool need_subprob, SortBufs &sb, const Launch &L);
+// The same with points ordered by fine-grid cell inside each bin.
+template <typename T>
+int run_fine_sort(int dim, const BinGeom &g, long long nbins, int M, const T *x, const T *y,
+                  const T *z, int maxsub, bool need_subprob, SortBufs &sb, const Launch &L);
 // Natural order (gpu_sort = 0): identity index and positions.
 template <typename T>
 int run_natural_order(int dim, const BinGeom &g, int M, const T *x, const T *y, const T *z,
@@ -59,6 +67,9 @@ int run_interp(int method, int dim, int ns, int M, const SortBufs &sb, cpx<T> *c
 
 // Bytes of LDS the SM kernels need for a tile (re and im of every cell).
 size_t sm_tile_bytes(int dim, int ns, const int *bs, size_t elem);
+// LDS bytes of the SM spread: 32-bit fixed-point tile plus staged strengths
+// for single precision (elem 4), FP64 tile for double precision.
+size_t spread_lds_bytes(int dim, int ns, const int *bs, size_t elem, int maxsub);
 
 template <typename T>
 int run_deconvolve(int dim, const int *ms, const int *nf, const cpx<T> *fw, cpx<T> *fk,
@@ -87,7 +98,8 @@ int interp_grad_dim(int ns, int M, const SortBufs &sb, cpx<T> *c, cpx<T> *dc, co
 size_t group_item_bytes(bool dbl);
 template <typename T, int DIM>
 int group_dim(int type, int ns, const void *items, int n, const long long *off,
-              const KerParams<T> &kp, const int *bs, long long jobs_bound, const Launch &L);
+              const KerParams<T> &kp, const int *bs, int maxsub, long long jobs_bound,
+              const Launch &L);
 template <typename T, int DIM>
 void group_item_fill(void *dst, const BinGeom &g, const SortBufs &sb, int maxsub, const void *in,
                      void *out, long long nftot, int M, int ntr);

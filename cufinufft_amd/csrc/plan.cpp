@@ -200,6 +200,82 @@ static void fseries(int nf, const Spread<T> &sp, std::vector<double> &out)
 }
 
 // ---------------------------------------------------------------------------
+// Piecewise-polynomial ES kernel for gpu_kerevalmeth = 1.  The reference
+// evaluates fixed fitted tables (contrib/ker_horner_allw_loop.c, sigma = 2
+// only); this library fits its own at plan time.  For a point whose first
+// tap lies at x1 = start - v in [-ns/2, -ns/2 + 1), tap i sits at offset
+// x1 + i; with z = 2(x1 + ns/2) - 1 in [-1, 1), phi on each tap's unit
+// interval is interpolated at deg+1 Chebyshev nodes and re-expanded in powers
+// of z (coef[k*ns + i] multiplies z^k for tap i).  The degree is the
+// smallest in [3, 24] whose error, sampled at 128 interior points per
+// interval, is at most max(tol/20, 4e-14) of the kernel peak e^beta.
+// Returns the degree and the achieved error relative to the peak.
+static int horner_fit(int ns, double beta, double c, double tol, std::vector<double> &coef,
+                      double *rel_err)
+{
+    const double hw = 0.5 * ns;
+    auto phi = [&](double t) {
+        const double a = 1.0 - c * t * t;
+        return std::fabs(t) <= hw ? std::exp(beta * std::sqrt(a > 0 ? a : 0.0)) : 0.0;
+    };
+    const double peak = std::exp(beta);
+    const double target = std::max(tol / 20.0, 4e-14);
+    int best_deg = 0;
+    double best_err = 1e300;
+    std::vector<double> C, f, a, tprev, tcur, tnext;
+    for (int deg = 3; deg <= 24; ++deg) {
+        const int K = deg + 1;
+        C.assign((size_t)K * ns, 0.0);
+        f.assign(K, 0.0);
+        a.assign(K, 0.0);
+        for (int i = 0; i < ns; ++i) {
+            for (int k = 0; k < K; ++k)
+                f[k] = phi(0.5 * (std::cos(M_PI * (k + 0.5) / K) + 1.0) - hw + i);
+            for (int j = 0; j < K; ++j) {
+                double s = 0;
+                for (int k = 0; k < K; ++k) s += f[k] * std::cos(M_PI * j * (k + 0.5) / K);
+                a[j] = (j == 0 ? 1.0 : 2.0) * s / K;
+            }
+            // sum_j a_j T_j(z) in powers of z, T_{j+1} = 2 z T_j - T_{j-1}
+            tprev.assign(K, 0.0);
+            tcur.assign(K, 0.0);
+            tprev[0] = 1.0;
+            C[i] += a[0];
+            if (K > 1) {
+                tcur[1] = 1.0;
+                C[(size_t)ns + i] += a[1];
+            }
+            for (int j = 2; j < K; ++j) {
+                tnext.assign(K, 0.0);
+                for (int m = 0; m < K; ++m)
+                    tnext[m] = (m > 0 ? 2.0 * tcur[m - 1] : 0.0) - tprev[m];
+                for (int m = 0; m <= j; ++m) C[(size_t)m * ns + i] += a[j] * tnext[m];
+                tprev.swap(tcur);
+                tcur.swap(tnext);
+            }
+        }
+        double err = 0;
+        for (int i = 0; i < ns; ++i) {
+            for (int s = 0; s < 128; ++s) {
+                const double z = -1.0 + 2.0 * (s + 0.5) / 128.0;
+                double v = C[(size_t)deg * ns + i];
+                for (int k = deg - 1; k >= 0; --k) v = v * z + C[(size_t)k * ns + i];
+                err = std::max(err, std::fabs(v - phi(0.5 * (z + 1.0) - hw + i)));
+            }
+        }
+        err /= peak;
+        if (err < best_err) {
+            best_err = err;
+            best_deg = deg;
+            coef = C;
+        }
+        if (err <= target) break;
+    }
+    if (rel_err) *rel_err = best_err;
+    return best_deg;
+}
+
+// ---------------------------------------------------------------------------
 struct DeviceGuard {
     int prev = -1;
     explicit DeviceGuard(int dev)
@@ -269,7 +345,15 @@ struct Plan {
     cpx<T> *fw = nullptr;
     T *fwker[3] = {nullptr, nullptr, nullptr};
     std::vector<double> fser[3];
+    // piecewise-polynomial kernel (gpu_kerevalmeth = 1): degree, FP64 host
+    // table, device tables (plain, then divided by e^beta)
+    int hdeg = 0;
+    double herr = 0;
+    std::vector<double> hcoef_host;
+    T *hcoef = nullptr;
     SortBufs sb;
+
+    KerParams<T> ker_params() const { return KerParams<T>{sp.beta, sp.c, sp.hw, hcoef, hdeg}; }
 
     hipfftHandle fft_full = 0, fft_rem = 0;
     int fft_rem_batch = 0;
@@ -328,6 +412,7 @@ struct Plan {
         fft_full_ok = fft_rem_ok = false;
         dfree(fw);
         for (auto &k : fwker) dfree(k);
+        dfree(hcoef);
         dfree(sb.binsize);
         dfree(sb.binstart);
         dfree(sb.substart);
@@ -339,6 +424,10 @@ struct Plan {
         dfree(sb.counters);
         if (sb.scan_part) hipFree(sb.scan_part);
         sb.scan_part = nullptr;
+        dfree(sb.kcount);
+        dfree(sb.kstart);
+        if (sb.kscan_part) hipFree(sb.kscan_part);
+        sb.kscan_part = nullptr;
     }
 
     int make_fft(hipfftHandle *h, int batch)
@@ -441,6 +530,7 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
     if (ntransf_ < 1) return kErrNdataNotValid;
     if (opts.gpu_method < 0 || opts.gpu_method > 2) return 2;
     if (opts.gpu_maxsubprobsize < 1) return 1;
+    if (opts.gpu_sort < 0 || opts.gpu_sort > 2) return 1;
     const int ier = setup_spreader<T>(sp, tol, (T)opts.upsampfac, opts.gpu_kerevalmeth);
     if (ier > 1) return ier;
     eps_warning = ier == kWarnEpsTooSmall;
@@ -473,7 +563,8 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
     // spread tile accumulates in FP64 whatever T is; the interp tile holds T.
     const int bsv[3] = {opts.gpu_binsizex, opts.gpu_binsizey, opts.gpu_binsizez};
     const size_t tile_bytes =
-        sm_tile_bytes(dim, sp.ns, bsv, type == 1 ? sizeof(double) : sizeof(T));
+        type == 1 ? spread_lds_bytes(dim, sp.ns, bsv, sizeof(T), opts.gpu_maxsubprobsize)
+                  : sm_tile_bytes(dim, sp.ns, bsv, sizeof(T));
     method = opts.gpu_method;
     if (method == 0) method = tile_bytes <= kMaxLdsBytes ? 2 : 1;
     if (method == 2 && tile_bytes > kMaxLdsBytes) return kErrLds;
@@ -512,11 +603,32 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
         if (e) return e;
         fft_full_ok = true;
     }
+    if (opts.gpu_kerevalmeth == 1) {
+        hdeg = horner_fit(sp.ns, (double)sp.beta, (double)sp.c, (double)tol, hcoef_host, &herr);
+        const size_t n = hcoef_host.size();
+        const double inv_peak = std::exp(-(double)sp.beta);
+        std::vector<T> h(2 * n);
+        for (size_t k = 0; k < n; ++k) {
+            h[k] = (T)hcoef_host[k];
+            h[n + k] = (T)(hcoef_host[k] * inv_peak);
+        }
+        if (dmalloc(&hcoef, h.size())) return kErrHip;
+        if (hipMemcpy(hcoef, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            return kErrHip;
+    }
     if (dmalloc(&sb.binsize, nbins) || dmalloc(&sb.binstart, nbins) ||
         dmalloc(&sb.substart, nbins + 1) || dmalloc(&sb.counters, 2))
         return kErrHip;
     const long long nparts = (nbins + kScanChunk - 1) / kScanChunk;
     if (hipMalloc(&sb.scan_part, sizeof(int2) * (size_t)nparts) != hipSuccess) return kErrHip;
+    if (opts.gpu_sort == 2) {  // fine (cell in bin) sort
+        const long long nkeys = nbins * (long long)geom.bs[0] * geom.bs[1] * geom.bs[2];
+        if (nkeys > 2147483647LL) return kErrNdataNotValid;
+        const long long kparts = (nkeys + kScanChunk - 1) / kScanChunk;
+        if (dmalloc(&sb.kcount, (size_t)nkeys) || dmalloc(&sb.kstart, (size_t)nkeys) ||
+            hipMalloc(&sb.kscan_part, sizeof(int2) * (size_t)kparts) != hipSuccess)
+            return kErrHip;
+    }
     return 0;
 }
 
@@ -558,6 +670,8 @@ int Plan<T>::setpts(int M_, const T *x, const T *y, const T *z)
     tbegin();
     if (method == 1 && !opts.gpu_sort) {
         e = run_natural_order<T>(dim, geom, M, kx, ky, kz, sb, L);
+    } else if (opts.gpu_sort == 2) {
+        e = run_fine_sort<T>(dim, geom, nbins, M, kx, ky, kz, maxsub, need_sub, sb, L);
     } else {
         e = run_bin_sort<T>(dim, geom, nbins, M, kx, ky, kz, maxsub, need_sub, sb, L);
     }
@@ -573,7 +687,7 @@ int Plan<T>::execute(cpx<T> *c, cpx<T> *fk)
     if (!pts_set) return 1;
     if ((M > 0 && !c) || !fk) return 1;
     DeviceGuard dg(opts.gpu_device_id);
-    KerParams<T> kp{sp.beta, sp.c, sp.hw};
+    const KerParams<T> kp = ker_params();
     const int maxsub = opts.gpu_maxsubprobsize;
     const T *kers[3] = {fwker[0], fwker[1], fwker[2]};
     int e = 0;
@@ -646,7 +760,7 @@ int Plan<T>::execute_grad(cpx<T> *c, cpx<T> *dc, cpx<T> *fk)
     if (type != 2 || !pts_set) return 1;
     if ((M > 0 && (!c || !dc)) || !fk) return 1;
     DeviceGuard dg(opts.gpu_device_id);
-    KerParams<T> kp{sp.beta, sp.c, sp.hw};
+    const KerParams<T> kp{sp.beta, sp.c, sp.hw, nullptr, 0};  // value + derivative: exp(sqrt()) form
     const T *kers[3] = {fwker[0], fwker[1], fwker[2]};
     const long long plane = (long long)M * ntransf;
     int e = 0;
@@ -714,6 +828,7 @@ struct Group {
     std::vector<cpx<T> *> c, fk;
     int type = 0, dim = 0, ns = 0, dev = 0;
     int bs[3] = {1, 1, 1};
+    int maxsub = 1;
     long long jobs_bound = 0;
     void *items = nullptr;
     long long *off = nullptr;
@@ -752,6 +867,7 @@ struct Group {
                 return 1;
             for (int d = 0; d < 3; ++d) bs[d] = std::max(bs[d], p->geom.bs[d]);
             jobs_bound += p->sub_bound * p->ntransf;
+            maxsub = std::max(maxsub, p->opts.gpu_maxsubprobsize);
             plans.push_back(ps[i]);
             c.push_back(static_cast<cpx<T> *>(cs[i]));
             fk.push_back(static_cast<cpx<T> *>(fks[i]));
@@ -794,7 +910,7 @@ struct Group {
     {
         int e = 0;
         const Plan<T> &p0 = *plans[0];
-        KerParams<T> kp{p0.sp.beta, p0.sp.c, p0.sp.hw};
+        const KerParams<T> kp = p0.ker_params();
         const int n = (int)plans.size();
         std::vector<hipStream_t> saved(n);
         for (int i = 0; i < n; ++i) {
@@ -803,10 +919,10 @@ struct Group {
         }
         auto spread_or_interp = [&]() {
             if (dim == 1)
-                return group_dim<T, 1>(type, ns, items, n, off, kp, bs, jobs_bound, L);
+                return group_dim<T, 1>(type, ns, items, n, off, kp, bs, maxsub, jobs_bound, L);
             if (dim == 2)
-                return group_dim<T, 2>(type, ns, items, n, off, kp, bs, jobs_bound, L);
-            return group_dim<T, 3>(type, ns, items, n, off, kp, bs, jobs_bound, L);
+                return group_dim<T, 2>(type, ns, items, n, off, kp, bs, maxsub, jobs_bound, L);
+            return group_dim<T, 3>(type, ns, items, n, off, kp, bs, maxsub, jobs_bound, L);
         };
         if (type == 1) {
             for (int i = 0; i < n && !e; ++i)
@@ -948,6 +1064,17 @@ int get_fseries_impl(P *p, double *k1, double *k2, double *k3)
     for (int d = 0; d < p->dim; ++d)
         if (o[d] && !p->fser[d].empty())
             std::memcpy(o[d], p->fser[d].data(), p->fser[d].size() * sizeof(double));
+    return 0;
+}
+
+template <typename P>
+int get_kernel_poly_impl(P *p, int *deg, double *rel_err, double *coef)
+{
+    if (!p) return 1;
+    if (deg) *deg = p->hdeg;
+    if (rel_err) *rel_err = p->herr;
+    if (coef && p->hdeg > 0)
+        std::memcpy(coef, p->hcoef_host.data(), p->hcoef_host.size() * sizeof(double));
     return 0;
 }
 
@@ -1143,6 +1270,15 @@ int cufinufft_get_fseries(cufinufft_plan plan, double *k1, double *k2, double *k
 int cufinufftf_get_fseries(cufinufftf_plan plan, double *k1, double *k2, double *k3)
 {
     return get_fseries_impl(plan, k1, k2, k3);
+}
+
+int cufinufft_get_kernel_poly(cufinufft_plan plan, int *deg, double *rel_err, double *coef)
+{
+    return get_kernel_poly_impl(plan, deg, rel_err, coef);
+}
+int cufinufftf_get_kernel_poly(cufinufftf_plan plan, int *deg, double *rel_err, double *coef)
+{
+    return get_kernel_poly_impl(plan, deg, rel_err, coef);
 }
 
 }  // extern "C"

@@ -60,7 +60,7 @@ int spread_ns(int method, int M, const SortBufs &sb, const cpx<T> *c, cpx<T> *fw
         k_spread_gm<T, DIM, NS><<<grid_for_points(M, L.ncu), 256, 0, L.stream>>>(
             M, sorted_pts(sb), c, fw, g, kp, ntr, nftot);
     } else {
-        const size_t lds = sm_tile_bytes(DIM, NS, g.bs, sizeof(double));  // FP64 accumulation
+        const size_t lds = spread_lds_bytes(DIM, NS, g.bs, sizeof(T), maxsub);
         if (lds > kMaxLdsBytes) return kErrLds;
         int grid = 0;
         int e = sm_grid(k_spread_sm<T, DIM, NS>, lds, sub_bound * ntr, L.ncu, &grid);
@@ -95,10 +95,11 @@ int interp_ns(int method, int M, const SortBufs &sb, cpx<T> *c, const cpx<T> *fw
 
 template <typename T, int DIM, int NS>
 int group_ns(int type, const GroupItem<T> *items, int n, const long long *off,
-             const KerParams<T> &kp, const int *bs, long long jobs_bound, const Launch &L)
+             const KerParams<T> &kp, const int *bs, int maxsub, long long jobs_bound,
+             const Launch &L)
 {
     if (type == 1) {
-        const size_t lds = sm_tile_bytes(DIM, NS, bs, sizeof(double));
+        const size_t lds = spread_lds_bytes(DIM, NS, bs, sizeof(T), maxsub);
         if (lds > kMaxLdsBytes) return kErrLds;
         int grid = 0;
         int e = sm_grid(k_spread_sm_group<T, DIM, NS>, lds, jobs_bound, L.ncu, &grid);
@@ -179,7 +180,8 @@ int interp_grad_dim(int ns, int M, const SortBufs &sb, cpx<T> *c, cpx<T> *dc, co
 
 template <typename T, int DIM>
 int group_dim(int type, int ns, const void *items, int n, const long long *off,
-              const KerParams<T> &kp, const int *bs, long long jobs_bound, const Launch &L)
+              const KerParams<T> &kp, const int *bs, int maxsub, long long jobs_bound,
+              const Launch &L)
 {
     const GroupItem<T> *it = static_cast<const GroupItem<T> *>(items);
     k_group_offsets<T><<<1, 64, 0, L.stream>>>(it, n, const_cast<long long *>(off));
@@ -187,7 +189,7 @@ int group_dim(int type, int ns, const void *items, int n, const long long *off,
     switch (ns) {
 #define CFA_CASE(N)                                                                          \
     case N:                                                                                  \
-        return group_ns<T, DIM, N>(type, it, n, off, kp, bs, jobs_bound, L);
+        return group_ns<T, DIM, N>(type, it, n, off, kp, bs, maxsub, jobs_bound, L);
         CFA_NS_CASES(CFA_CASE)
 #undef CFA_CASE
     }
@@ -213,8 +215,8 @@ void group_item_fill(void *dst, const BinGeom &g, const SortBufs &sb, int maxsub
 
 #define CFA_INSTANTIATE_SPREADINTERP(T, DIM)                                                  \
     template int cfa::group_dim<T, DIM>(int, int, const void *, int, const long long *,      \
-                                        const cfa::KerParams<T> &, const int *, long long,   \
-                                        const cfa::Launch &);                                \
+                                        const cfa::KerParams<T> &, const int *, int,         \
+                                        long long, const cfa::Launch &);                     \
     template void cfa::group_item_fill<T, DIM>(void *, const cfa::BinGeom &,                 \
                                                const cfa::SortBufs &, int, const void *,     \
                                                void *, long long, int, int);                 \

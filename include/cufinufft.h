@@ -140,6 +140,16 @@ int cufinufftf_get_timing(cufinufftf_plan plan, double *ms, int *count);
 int cufinufft_get_fseries(cufinufft_plan plan, double *k1, double *k2, double *k3);
 int cufinufftf_get_fseries(cufinufftf_plan plan, double *k1, double *k2, double *k3);
 
+/* Piecewise-polynomial kernel of a plan made with gpu_kerevalmeth = 1 (no
+ * counterpart in the reference, whose Horner tables are compiled in,
+ * contrib/ker_horner_allw_loop.c): *deg receives the degree (0 when the plan
+ * evaluates exp(sqrt())), *rel_err the fit's sampled maximum error relative
+ * to the kernel peak e^beta, and coef (if not NULL, (deg+1)*ns doubles) the
+ * FP64 coefficients, coef[k*ns + i] multiplying z^k for tap i, where
+ * z = 2(start - x) + ns - 1 and start = ceil(x - ns/2). */
+int cufinufft_get_kernel_poly(cufinufft_plan plan, int *deg, double *rel_err, double *coef);
+int cufinufftf_get_kernel_poly(cufinufftf_plan plan, int *deg, double *rel_err, double *coef);
+
 #ifdef __cplusplus
 }
 #endif

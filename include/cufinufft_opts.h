@@ -13,7 +13,8 @@
 typedef struct cufinufft_opts {
     double upsampfac;        /* sigma; 2.0 is the standard choice */
     int gpu_method;          /* 0 auto (autotuned), 1 GM / GM-sort, 2 SM subproblems */
-    int gpu_sort;            /* method 1 only: 0 natural order, 1 bin-sorted order */
+    int gpu_sort;            /* 0 natural order (method 1 only), 1 bin-sorted order,
+                                2 bin-sorted and by fine-grid cell inside each bin */
     int gpu_binsizex;        /* bin (subproblem tile) size, -1 = default */
     int gpu_binsizey;
     int gpu_binsizez;
