@@ -44,7 +44,8 @@ def candidates(nufft_type, dim):
         bins = [(32, 32, 1), (16, 16, 1), (64, 32, 1), (64, 64, 1), (32, 16, 1)]
     else:
         bins = [(1024, 1, 1), (512, 1, 1), (2048, 1, 1)]
-    subs = (256, 1024, 4096)
+    # type 2: subproblems of >= 512 points run 512-thread workgroups
+    subs = (256, 1024, 4096) if nufft_type == 1 else (256, 512, 1024, 4096)
     out = []
     for b in bins:
         for msub in subs:
@@ -59,6 +60,22 @@ def candidates(nufft_type, dim):
     for b in bins[:2]:
         out.append(dict(gpu_method=1, gpu_binsizex=b[0], gpu_binsizey=b[1], gpu_binsizez=b[2],
                         gpu_maxsubprobsize=1024))
+    # owner-computes blocks (type 1): halo-free tiles, so blocks can be large;
+    # blocks that do not tile the fine grid are rejected by makeplan
+    if nufft_type == 1:
+        if dim == 3:
+            blocks = [(16, 16, 8), (16, 16, 16), (32, 16, 8), (32, 32, 8), (32, 32, 16)]
+        elif dim == 2:
+            blocks = [(64, 64, 1), (128, 64, 1), (128, 128, 1), (256, 64, 1)]
+        else:
+            blocks = [(2048, 1, 1), (4096, 1, 1), (8192, 1, 1)]
+        for b in blocks:
+            # blocks own their cells: large subproblems (64-bit fixed point
+            # in FP32) keep the atomic flushes of hot blocks rare
+            for msub in (1024, 4096, 16384):
+                out.append(dict(gpu_method=4, gpu_obinsizex=b[0], gpu_obinsizey=b[1],
+                                gpu_obinsizez=b[2], gpu_binsizex=b[0], gpu_binsizey=b[1],
+                                gpu_binsizez=b[2], gpu_maxsubprobsize=msub))
     return out
 
 

@@ -35,6 +35,9 @@ struct KerParams {
     // holding the kernel divided by e^beta (peak 1).
     const T *hc;
     int hdeg;
+    // FP32 SM spread: 1 accumulates the LDS tile in 64-bit fixed point, 0 in
+    // 32-bit (chosen per plan from tol and the subproblem size)
+    int fxwide;
 };
 
 // Points as the execute kernels read them, in bin-sorted order: idx[i] is
@@ -141,8 +144,6 @@ __device__ __forceinline__ double tap_pos(double v, int n)
     return fmin(fmax(v, -1.0), (double)n + 1.0);
 }
 
-// First tap index and the ns kernel values of one axis:
-// start = ceil(v - ns/2) (exact in double), ker[i] = phi(start + i - v).
 // Horner evaluation of the ns per-tap polynomials at z (all taps of one
 // point share z); coefficient addresses are lane-uniform.
 template <typename T, int NS>
@@ -156,6 +157,9 @@ __device__ __forceinline__ void horner_taps(T z, const T *__restrict__ hc, int d
     }
 }
 
+// First tap index and the ns kernel values of one axis:
+// start = ceil(v - ns/2) (exact in double), ker[i] = phi(start + i - v),
+// by exp(sqrt()) or by the polynomial table.
 template <typename T, int NS>
 __device__ __forceinline__ int taps_global(double v, const KerParams<T> &kp, T *ker)
 {
@@ -777,24 +781,45 @@ __device__ __forceinline__ float block_sum256(float v, float *scratch)
     return s;
 }
 
-// LDS bytes of the single-precision fixed-point spread: 32-bit re / im per
-// tile cell, the job's strengths staged as complex floats, and 4 floats of
-// reduction scratch.
-__host__ __device__ inline size_t fx_spread_lds(size_t cells, int maxsub)
+// LDS bytes of the single-precision fixed-point spread: 32- or 64-bit re / im
+// per tile cell, the job's strengths staged as complex floats, and 4 floats
+// of reduction scratch.
+__host__ __device__ inline size_t fx_spread_lds(size_t cells, int maxsub, int wide)
 {
-    return cells * 8 + (size_t)maxsub * 8 + 16;
+    return cells * (wide ? 16 : 8) + (size_t)maxsub * 8 + 16;
 }
 
-// Single-precision SM spread job with 32-bit fixed-point LDS accumulation.
-// On gfx950 ds_add_u32 sustains 8-14 lane-ops per CU clock against 2.5 for
-// ds_add_f64 and 0.33 for ds_add_f32 (measured).  Exactness by construction:
-// with the kernel normalised to phi/exp(beta) <= 1 per axis, no tile cell can
-// exceed sum_p (|re c_p| + |im c_p|) =: A over the job's points, so the job
-// scales contributions by the power of two S = 2^30 / A (rounded down), well
-// inside int32; each contribution is rounded to nearest, an absolute error of
-// at most A * 2^-31 per tap, below FP32 rounding of the same sums.  The flush
-// multiplies by exp(DIM beta) / S.
-template <int DIM, int NS>
+// Fixed-point accumulator widths: headroom bits of the tile scale, the
+// round-to-nearest conversion and the LDS atomic (ds_add_u32 / ds_add_u64;
+// two's-complement sums are exact modulo 2^bits, and the scale keeps every
+// final cell value inside the signed range).
+template <typename Acc> struct FxAcc;
+template <> struct FxAcc<int> {
+    static constexpr int kBits = 30;
+    __device__ static int cvt(float v) { return __float2int_rn(v); }
+    __device__ static void add(int *p, int v) { atomicAdd(p, v); }
+};
+template <> struct FxAcc<long long> {
+    static constexpr int kBits = 62;
+    __device__ static long long cvt(float v) { return __float2ll_rn(v); }
+    __device__ static void add(long long *p, long long v)
+    {
+        atomicAdd(reinterpret_cast<unsigned long long *>(p), (unsigned long long)v);
+    }
+};
+
+// Single-precision SM spread job with fixed-point LDS accumulation (Acc =
+// int or long long).  On gfx950 ds_add_u32 sustains 8-14 lane-ops per CU
+// clock, ds_add_u64 4.9, against 2.5 for ds_add_f64 and 0.33 for ds_add_f32
+// (measured).  Exactness by construction: with the kernel normalised to
+// phi/exp(beta) <= 1 per axis, no tile cell can exceed
+// sum_p (|re c_p| + |im c_p|) =: A over the job's points, so the job scales
+// contributions by the power of two S = 2^kBits / A (rounded down), inside
+// the signed range; each contribution is rounded to nearest, an absolute
+// error of at most A * 2^-(kBits+1) per tap.  With 32 bits that error grows
+// with the points per job (A); the plan picks 64 bits when it would matter
+// at its tolerance.  The flush multiplies by exp(DIM beta) / S in FP64.
+template <typename Acc, int DIM, int NS>
 __device__ __forceinline__ void spread_sm_job_fx(const SubprobJob &J, int M, const SortedPts &sp,
                                                  const cpx<float> *__restrict__ c,
                                                  cpx<float> *__restrict__ fw, const BinGeom &g,
@@ -802,9 +827,9 @@ __device__ __forceinline__ void spread_sm_job_fx(const SubprobJob &J, int M, con
                                                  unsigned char *smem, int maxsub)
 {
     const Tile<DIM, NS> tl(g, J.b);
-    int *sre = reinterpret_cast<int *>(smem);
-    int *sim = sre + tl.n;
-    cpx<float> *cst = reinterpret_cast<cpx<float> *>(sim + tl.n);  // 8*tl.n bytes in: aligned
+    Acc *sre = reinterpret_cast<Acc *>(smem);
+    Acc *sim = sre + tl.n;
+    cpx<float> *cst = reinterpret_cast<cpx<float> *>(sim + tl.n);  // 2*sizeof(Acc)*tl.n bytes in: aligned
     float *red = reinterpret_cast<float *>(cst + maxsub);
     for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
         sre[q] = 0;
@@ -819,7 +844,7 @@ __device__ __forceinline__ void spread_sm_job_fx(const SubprobJob &J, int M, con
     const float A = block_sum256(part, red);  // also orders the stores above
     int ex = 0;
     frexpf(A, &ex);                            // A < 2^ex
-    const int sh = A > 0.0f ? 30 - ex : 0;     // S = 2^sh, applied with ldexp (no overflow)
+    const int sh = A > 0.0f ? FxAcc<Acc>::kBits - ex : 0;  // S = 2^sh, applied with ldexp
     for (int q = threadIdx.x; q < J.np; q += blockDim.x) {
         const int i = J.p0 + q;
         float k1[NS], k2[NS], k3[NS];
@@ -839,23 +864,23 @@ __device__ __forceinline__ void spread_sm_job_fx(const SubprobJob &J, int M, con
                 const int row = ((l3 + dz) * tl.tw[1] + (l2 + dy)) * tl.tw[0] + l1;
 #pragma unroll
                 for (int a = 0; a < NS; ++a) {
-                    atomicAdd(&sre[row + a], __float2int_rn(cr * k1[a]));
-                    atomicAdd(&sim[row + a], __float2int_rn(ci * k1[a]));
+                    FxAcc<Acc>::add(&sre[row + a], FxAcc<Acc>::cvt(cr * k1[a]));
+                    FxAcc<Acc>::add(&sim[row + a], FxAcc<Acc>::cvt(ci * k1[a]));
                 }
             }
         }
     }
     __syncthreads();
-    // e^(DIM*beta) in FP64: in FP32 the rounding of DIM*beta (~50) alone
-    // would scale the whole tile by up to 2e-6
-    const float back = (float)ldexp(exp((double)DIM * (double)kp.beta), -sh);
+    // e^(DIM*beta) / S in FP64: in FP32 the rounding of DIM*beta (~50) alone
+    // would scale the whole tile by up to 2e-6, and 2^-sh can leave FP32 range
+    const double back = ldexp(exp((double)DIM * (double)kp.beta), -sh);
     cpx<float> *f = fw + (long long)J.t * nftot;
     for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
-        const int ir = sre[q], ii = sim[q];
+        const Acc ir = sre[q], ii = sim[q];
         if (ir != 0 || ii != 0) {
             cpx<float> *p = f + tl.grid_offset(q, g);
-            gadd(&p->x, (float)ir * back);
-            gadd(&p->y, (float)ii * back);
+            gadd(&p->x, (float)((double)ir * back));
+            gadd(&p->y, (float)((double)ii * back));
         }
     }
     __syncthreads();
@@ -923,10 +948,14 @@ k_spread_sm(int M, SortedPts sp, const cpx<T> *__restrict__ c, cpx<T> *__restric
     const int total = sa.counters[1];
     const JobRange jr = xcd_jobs((long long)total * ntr);
     for (long long job = jr.begin; job < jr.end; job += jr.step) {
-        if constexpr (sizeof(T) == 4)
-            spread_sm_job_fx<DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g, kp, nftot,
-                                      smem, sa.maxsub);
-        else
+        if constexpr (sizeof(T) == 4) {
+            if (kp.fxwide)
+                spread_sm_job_fx<long long, DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g,
+                                                     kp, nftot, smem, sa.maxsub);
+            else
+                spread_sm_job_fx<int, DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g, kp,
+                                               nftot, smem, sa.maxsub);
+        } else
             spread_sm_job<T, DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g, kp, nftot,
                                       smem);
     }
@@ -987,11 +1016,315 @@ k_spread_sm_group(const GroupItem<T> *__restrict__ items, int n,
         const int p = group_member(off, n, job);
         const GroupItem<T> &it = items[p];
         const SubprobJob J = subprob_job(job - off[p], it.sa.counters[1], it.sa);
-        if constexpr (sizeof(T) == 4)
-            spread_sm_job_fx<DIM, NS>(J, it.M, it.sp, it.in, it.out, it.g, kp, it.nftot, smem,
-                                      it.sa.maxsub);
-        else
+        if constexpr (sizeof(T) == 4) {
+            if (kp.fxwide)
+                spread_sm_job_fx<long long, DIM, NS>(J, it.M, it.sp, it.in, it.out, it.g, kp,
+                                                     it.nftot, smem, it.sa.maxsub);
+            else
+                spread_sm_job_fx<int, DIM, NS>(J, it.M, it.sp, it.in, it.out, it.g, kp, it.nftot,
+                                               smem, it.sa.maxsub);
+        } else
             spread_sm_job<T, DIM, NS>(J, it.M, it.sp, it.in, it.out, it.g, kp, it.nftot, smem);
+    }
+}
+
+// ===========================================================================
+// Method 4: owner-computes block spread.  The role is that of the reference's
+// block-gather method (spread3d_wrapper.cu:393-889: output bins, points
+// replicated into ghost regions, no halo atomics); the design here is this
+// library's own.  The fine grid is cut into blocks of ob_x x ob_y x ob_z
+// cells (nf % ob == 0, as the reference requires).  setpts places one copy
+// of each point in every block that its ns-wide support touches, sorted by
+// (block, in-block cell) so that a block's copies are in cell order; each
+// copy stores its position shifted by a multiple of nf into the block's own
+// periodic frame.  execute accumulates, per block, only the taps inside the
+// block in a halo-free LDS tile and writes the block with plain stores when
+// one subproblem covers it (no other job touches those cells), else adds
+// the bounding box its taps touched with global atomics.  Inside a block the
+// copies stay in arrival order: cell order would put the 64 lanes of a wave
+// on the same cells in dense clusters, and same-address LDS atomics
+// serialise (measured: 15x the address conflicts of arrival order).
+// ===========================================================================
+
+__device__ __forceinline__ int floor_div(int a, int b)
+{
+    return a >= 0 ? a / b : -((-a + b - 1) / b);
+}
+
+// Along one axis: first tap and the unwrapped block frames kb0..kb1 touched.
+__device__ __forceinline__ void blk_frames(double v, int ns, int ob, int &start, int &kb0,
+                                           int &kb1)
+{
+    start = (int)ceil(v - 0.5 * ns);
+    kb0 = floor_div(start, ob);
+    kb1 = floor_div(start + ns - 1, ob);
+}
+
+// Largest number of copies of one point: blocks touched per axis.
+__host__ __device__ inline int blk_max_copies(int dim, int ns, const int *ob)
+{
+    int c = 1;
+    for (int d = 0; d < dim; ++d) c *= (ns - 2 + ob[d]) / ob[d] + 1;
+    return c;
+}
+
+// Visits every copy of point i: f(block, axis positions shifted into the
+// block's frame).
+template <int DIM, typename F>
+__device__ __forceinline__ void blk_copies(const double *v, int ns, const BinGeom &g, F f)
+{
+    int st[3] = {0, 0, 0}, k0[3] = {0, 0, 0}, k1[3] = {0, 0, 0};
+    for (int d = 0; d < DIM; ++d) blk_frames(v[d], ns, g.bs[d], st[d], k0[d], k1[d]);
+    for (int kz = k0[2]; kz <= k1[2]; ++kz)
+        for (int ky = k0[1]; ky <= k1[1]; ++ky)
+            for (int kx = k0[0]; kx <= k1[0]; ++kx) {
+                const int kb[3] = {kx, ky, kz};
+                int B[3] = {0, 0, 0};
+                double sv[3] = {0, 0, 0};
+                for (int d = 0; d < DIM; ++d) {
+                    B[d] = wrapmod(kb[d], g.nb[d]);
+                    sv[d] = v[d] + (double)(B[d] - kb[d]) * g.bs[d];  // exact: a multiple of nf
+                }
+                f(B[0] + g.nb[0] * (B[1] + g.nb[1] * B[2]), sv);
+            }
+}
+
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+k_blk_count(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
+            BinGeom g, int ns, int *__restrict__ blkcount)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+        double v[3];
+        v[0] = tap_pos(rescale_exact(x[i], g.nf[0]), g.nf[0]);
+        if (DIM > 1) v[1] = tap_pos(rescale_exact(y[i], g.nf[1]), g.nf[1]);
+        if (DIM > 2) v[2] = tap_pos(rescale_exact(z[i], g.nf[2]), g.nf[2]);
+        blk_copies<DIM>(v, ns, g, [&](int blk, const double *) { atomicAdd(&blkcount[blk], 1); });
+    }
+}
+
+// Scatter of the copies: slot = blkstart[blk] + (remaining count - 1); the
+// counts are consumed (each reaches 0).
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+k_blk_scatter(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
+              BinGeom g, int ns, const int *__restrict__ blkstart, int *__restrict__ blkcount,
+              int *__restrict__ idx, double *__restrict__ pos, long long stride)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
+        double v[3];
+        v[0] = tap_pos(rescale_exact(x[i], g.nf[0]), g.nf[0]);
+        if (DIM > 1) v[1] = tap_pos(rescale_exact(y[i], g.nf[1]), g.nf[1]);
+        if (DIM > 2) v[2] = tap_pos(rescale_exact(z[i], g.nf[2]), g.nf[2]);
+        blk_copies<DIM>(v, ns, g, [&](int blk, const double *sv) {
+            const int s = blkstart[blk] + atomicSub(&blkcount[blk], 1) - 1;
+            idx[s] = i;
+            for (int d = 0; d < DIM; ++d) pos[d * stride + s] = sv[d];
+        });
+    }
+}
+
+// Kernel values of one axis at the taps start..start+NS-1 of a point at v,
+// optionally divided by e^beta (peak 1, for the fixed-point tiles).
+template <typename T, int NS, bool NORM>
+__device__ __forceinline__ int taps_blk(double v, const KerParams<T> &kp, T *ker)
+{
+    const int start = (int)ceil(v - 0.5 * NS);
+    if (kp.hdeg > 0) {
+        horner_taps<T, NS>((T)(2.0 * ((double)start - v) + (NS - 1)),
+                           NORM ? kp.hc + (kp.hdeg + 1) * NS : kp.hc, kp.hdeg, ker);
+        return start;
+    }
+    const T x1 = (T)((double)start - v);
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        const T t = x1 + (T)i;
+        const T a = fmax(T(1) - kp.c * t * t, T(0));
+        ker[i] = fabs(t) < kp.hw ? exp(kp.beta * (NORM ? sqrt(a) - T(1) : sqrt(a))) : T(0);
+    }
+    return start;
+}
+
+// LDS bytes of a method-4 job: re / im accumulators per block cell (Acc of
+// accsize bytes), reduction scratch and the bounding box.  Strengths are not
+// staged (the fixed-point scale pass gathers them once more, from L2), so
+// large subproblems cost no LDS.
+__host__ __device__ inline size_t blk_lds(size_t cells, int accsize)
+{
+    return cells * 2 * accsize + 48;
+}
+
+template <typename Acc> struct BlkAcc;
+template <> struct BlkAcc<double> {
+    __device__ static double cvt(double v) { return v; }
+    __device__ static void add(double *p, double v) { ladd(p, v); }
+};
+template <> struct BlkAcc<int> {
+    __device__ static int cvt(float v) { return __float2int_rn(v); }
+    __device__ static void add(int *p, int v) { atomicAdd(p, v); }
+};
+template <> struct BlkAcc<long long> {
+    __device__ static long long cvt(float v) { return __float2ll_rn(v); }
+    __device__ static void add(long long *p, long long v)
+    {
+        atomicAdd(reinterpret_cast<unsigned long long *>(p), (unsigned long long)v);
+    }
+};
+
+// One method-4 job: the copies J.p0 .. J.p0+J.np-1 of block J.b.  Acc is
+// double for FP64 plans, int / long long (fixed point, as in the SM spread)
+// for FP32 plans.
+template <typename T, typename Acc, int DIM, int NS>
+__device__ __forceinline__ void spread_blk_job(const SubprobJob &J, int nsub, int M,
+                                               const SortedPts &sp,
+                                               const cpx<T> *__restrict__ c,
+                                               cpx<T> *__restrict__ fw, const BinGeom &g,
+                                               const KerParams<T> &kp, long long nftot,
+                                               unsigned char *smem, int maxsub)
+{
+    constexpr bool FX = sizeof(T) == 4;
+    const int wx = g.bs[0], wy = g.bs[1], wz = g.bs[2];
+    const int ox = (J.b % g.nb[0]) * wx;
+    const int oy = ((J.b / g.nb[0]) % g.nb[1]) * wy;
+    const int oz = (J.b / (g.nb[0] * g.nb[1])) * wz;
+    const int n = wx * wy * wz;
+    Acc *sre = reinterpret_cast<Acc *>(smem);
+    Acc *sim = sre + n;
+    float *red = reinterpret_cast<float *>(sim + n);
+    int *box = reinterpret_cast<int *>(red + 4);
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        sre[q] = 0;
+        sim[q] = 0;
+    }
+    if (threadIdx.x < 3) {
+        box[threadIdx.x] = 1 << 30;      // min corner
+        box[3 + threadIdx.x] = -1;       // max corner
+    }
+    int sh = 0;
+    if constexpr (FX) {
+        float part = 0.0f;
+        for (int q = threadIdx.x; q < J.np; q += blockDim.x) {
+            const cpx<float> cv = c[(long long)J.t * M + sp.idx[J.p0 + q]];
+            part += fabsf(cv.x) + fabsf(cv.y);
+        }
+        const float A = block_sum256(part, red);  // also orders the tile zeroing
+        int ex = 0;
+        frexpf(A, &ex);
+        sh = A > 0.0f ? (sizeof(Acc) == 4 ? 30 : 62) - ex : 0;
+    } else {
+        __syncthreads();
+    }
+    const int w[3] = {wx, wy, wz}, o[3] = {ox, oy, oz};
+    int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-1, -1, -1};  // touched cells
+    for (int q = threadIdx.x; q < J.np; q += blockDim.x) {
+        const int i = J.p0 + q;
+        T k[3][NS];
+        int l[3] = {0, 0, 0}, a0[3] = {0, 0, 0}, a1[3] = {1, 1, 1};
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            l[d] = taps_blk<T, NS, FX>(sp.pos[d * sp.stride + i], kp, k[d]) - o[d];
+            a0[d] = l[d] < 0 ? -l[d] : 0;
+            a1[d] = w[d] - l[d] < NS ? w[d] - l[d] : NS;
+            lo[d] = min(lo[d], l[d] + a0[d]);
+            hi[d] = max(hi[d], l[d] + a1[d] - 1);
+        }
+        const cpx<T> cv = c[(long long)J.t * M + sp.idx[i]];
+        T sr = cv.x, si = cv.y;
+        if constexpr (FX) {
+            sr = ldexpf(cv.x, sh);
+            si = ldexpf(cv.y, sh);
+        }
+#pragma unroll
+        for (int dz = 0; dz < (DIM > 2 ? NS : 1); ++dz) {
+            if (dz < a0[2] || dz >= a1[2]) continue;
+            const T kz = DIM > 2 ? k[2][dz] : T(1);
+#pragma unroll
+            for (int dy = 0; dy < (DIM > 1 ? NS : 1); ++dy) {
+                if (dy < a0[1] || dy >= a1[1]) continue;
+                const T kyz = (DIM > 1 ? k[1][dy] : T(1)) * kz;
+                const T cr = sr * kyz, ci = si * kyz;
+                const int row = ((l[2] + dz) * wy + (l[1] + dy)) * wx + l[0];
+#pragma unroll
+                for (int a = 0; a < NS; ++a) {
+                    if (a >= a0[0] && a < a1[0]) {
+                        BlkAcc<Acc>::add(&sre[row + a], BlkAcc<Acc>::cvt(cr * k[0][a]));
+                        BlkAcc<Acc>::add(&sim[row + a], BlkAcc<Acc>::cvt(ci * k[0][a]));
+                    }
+                }
+            }
+        }
+    }
+    if (nsub > 1) {
+        // bounding box of the touched cells: wave reductions, then one LDS
+        // atomic per wave (same-address LDS atomics serialise across lanes)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                lo[d] = min(lo[d], __shfl_xor(lo[d], off, 64));
+                hi[d] = max(hi[d], __shfl_xor(hi[d], off, 64));
+            }
+            if ((threadIdx.x & 63) == 0) {
+                atomicMin(&box[d], lo[d]);
+                atomicMax(&box[3 + d], hi[d]);
+            }
+        }
+    }
+    __syncthreads();
+    // e^(DIM*beta) / 2^sh in FP64 for the fixed-point tiles (see the SM spread)
+    const double back = FX ? ldexp(exp((double)DIM * (double)kp.beta), -sh) : 1.0;
+    cpx<T> *f = fw + (long long)J.t * nftot;
+    if (nsub == 1) {
+        // the only job of this block: plain coalesced stores of every cell
+        for (int q = threadIdx.x; q < n; q += blockDim.x) {
+            const int qx = q % wx, qy = (q / wx) % wy, qz = q / (wx * wy);
+            cpx<T> v;
+            v.x = (T)((double)sre[q] * back);
+            v.y = (T)((double)sim[q] * back);
+            f[((long long)(oz + qz) * g.nf[1] + (oy + qy)) * g.nf[0] + ox + qx] = v;
+        }
+    } else {
+        const int x0 = box[0], y0 = DIM > 1 ? box[1] : 0, z0 = DIM > 2 ? box[2] : 0;
+        const int bw = box[3] - x0 + 1;
+        const int bh = DIM > 1 ? box[4] - y0 + 1 : 1;
+        const int bd = DIM > 2 ? box[5] - z0 + 1 : 1;
+        const int nb = bw > 0 && bh > 0 && bd > 0 ? bw * bh * bd : 0;
+        for (int q = threadIdx.x; q < nb; q += blockDim.x) {
+            const int qx = x0 + q % bw, qy = y0 + (q / bw) % bh, qz = z0 + q / (bw * bh);
+            const int t = (qz * wy + qy) * wx + qx;
+            const Acc ir = sre[t], ii = sim[t];
+            if (ir != 0 || ii != 0) {
+                cpx<T> *p = f + ((long long)(oz + qz) * g.nf[1] + (oy + qy)) * g.nf[0] + ox + qx;
+                gadd(&p->x, (T)((double)ir * back));
+                gadd(&p->y, (T)((double)ii * back));
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int DIM, int NS>
+__global__ void __launch_bounds__(256)
+k_spread_blk(int M, SortedPts sp, const cpx<T> *__restrict__ c, cpx<T> *__restrict__ fw,
+             BinGeom g, KerParams<T> kp, SubprobArgs sa, int ntr, long long nftot)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int total = sa.counters[1];
+    const JobRange jr = xcd_jobs((long long)total * ntr);
+    for (long long job = jr.begin; job < jr.end; job += jr.step) {
+        const SubprobJob J = subprob_job(job, total, sa);
+        const int nsub = sa.substart[J.b + 1] - sa.substart[J.b];
+        if constexpr (sizeof(T) == 4) {
+            if (kp.fxwide)
+                spread_blk_job<T, long long, DIM, NS>(J, nsub, M, sp, c, fw, g, kp, nftot, smem,
+                                                      sa.maxsub);
+            else
+                spread_blk_job<T, int, DIM, NS>(J, nsub, M, sp, c, fw, g, kp, nftot, smem,
+                                                sa.maxsub);
+        } else {
+            spread_blk_job<T, double, DIM, NS>(J, nsub, M, sp, c, fw, g, kp, nftot, smem,
+                                               sa.maxsub);
+        }
     }
 }
 
@@ -1045,8 +1378,11 @@ __device__ __forceinline__ void interp_sm_job(const SubprobJob &J, int M, const 
     __syncthreads();
 }
 
-template <typename T, int DIM, int NS>
-__global__ void __launch_bounds__(256)
+// BT threads per workgroup: 512 (8 waves over one LDS tile) doubles the
+// waves per CU at the same LDS footprint when subproblems hold >= 512 points;
+// the FP64 interp is latency-bound at 2 tiles per CU otherwise.
+template <typename T, int DIM, int NS, int BT>
+__global__ void __launch_bounds__(BT)
 k_interp_sm(int M, SortedPts sp, cpx<T> *__restrict__ c, const cpx<T> *__restrict__ fw,
             BinGeom g, KerParams<T> kp, SubprobArgs sa, int ntr, long long nftot)
 {

@@ -53,9 +53,18 @@ int run_fine_sort(int dim, const BinGeom &g, long long nbins, int M, const T *x,
 template <typename T>
 int run_natural_order(int dim, const BinGeom &g, int M, const T *x, const T *y, const T *z,
                       SortBufs &sb, const Launch &L);
+// Method 4 (owner-computes blocks, g.bs = block size): count and scan the
+// point copies per (block, cell) key (total copies -> counters[0]), then
+// scatter them and build the per-block subproblems.
+template <typename T>
+int run_blk_count(int dim, const BinGeom &g, int ns, int M, const T *x, const T *y, const T *z,
+                  SortBufs &sb, const Launch &L);
+template <typename T>
+int run_blk_scatter(int dim, const BinGeom &g, int ns, int M, int ncopies, const T *x,
+                    const T *y, const T *z, int maxsub, SortBufs &sb, const Launch &L);
 
 // Spread / interp of ntr transforms over the sorted points of sb.
-// method 1 = GM(-sort), 2 = SM.
+// method 1 = GM(-sort), 2 = SM, 4 = owner-computes blocks (spread only).
 template <typename T>
 int run_spread(int method, int dim, int ns, int M, const SortBufs &sb, const cpx<T> *c,
                cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp, int maxsub,
@@ -69,7 +78,9 @@ int run_interp(int method, int dim, int ns, int M, const SortBufs &sb, cpx<T> *c
 size_t sm_tile_bytes(int dim, int ns, const int *bs, size_t elem);
 // LDS bytes of the SM spread: 32-bit fixed-point tile plus staged strengths
 // for single precision (elem 4), FP64 tile for double precision.
-size_t spread_lds_bytes(int dim, int ns, const int *bs, size_t elem, int maxsub);
+size_t spread_lds_bytes(int dim, int ns, const int *bs, size_t elem, int maxsub, int fxwide);
+// LDS bytes of a method-4 (owner-computes block) spread job over blocks ob.
+size_t blk_lds_bytes(int dim, const int *ob, size_t elem, int maxsub, int fxwide);
 
 template <typename T>
 int run_deconvolve(int dim, const int *ms, const int *nf, const cpx<T> *fw, cpx<T> *fk,

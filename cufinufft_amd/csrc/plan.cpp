@@ -199,6 +199,17 @@ static void fseries(int nf, const Spread<T> &sp, std::vector<double> &out)
     }
 }
 
+// Accumulator width of the FP32 fixed-point spread tile.  A 32-bit tile
+// rounds every tap to A * 2^-31 with A the strength mass of the job, so its
+// error grows with the points per subproblem: CPU emulation of the tile on a
+// clustered 3D tol-1e-6 case adds about 2-3 * maxsub * 2^-30 to the relative
+// error of the transform.  The 32-bit tile is kept while 4 * maxsub * 2^-30
+// stays within tol/5; otherwise the tile is 64-bit (ds_add_u64).
+static int fx_wide(int maxsub, double tol)
+{
+    return 4.0 * (double)maxsub * std::ldexp(1.0, -30) > tol / 5.0 ? 1 : 0;
+}
+
 // ---------------------------------------------------------------------------
 // Piecewise-polynomial ES kernel for gpu_kerevalmeth = 1.  The reference
 // evaluates fixed fitted tables (contrib/ker_horner_allw_loop.c, sigma = 2
@@ -353,7 +364,12 @@ struct Plan {
     T *hcoef = nullptr;
     SortBufs sb;
 
-    KerParams<T> ker_params() const { return KerParams<T>{sp.beta, sp.c, sp.hw, hcoef, hdeg}; }
+    int fxwide = 0;  // FP32 SM spread tile in 64-bit (1) or 32-bit (0) fixed point
+
+    KerParams<T> ker_params() const
+    {
+        return KerParams<T>{sp.beta, sp.c, sp.hw, hcoef, hdeg, fxwide};
+    }
 
     hipfftHandle fft_full = 0, fft_rem = 0;
     int fft_rem_batch = 0;
@@ -466,6 +482,8 @@ struct Plan {
     int makeplan(int type_, int dim_, const int *nmodes, int iflag_, int ntransf_, T tol,
                  int maxbatch_, const cufinufft_opts *o);
     int setpts(int M_, const T *x, const T *y, const T *z);
+    int setpts_blocks();
+    int cap_copies = -1;  // capacity of idx / pos in point copies (method 4)
     int execute(cpx<T> *c, cpx<T> *fk);
     int execute_grad(cpx<T> *c, cpx<T> *dc, cpx<T> *fk);
 };
@@ -492,9 +510,25 @@ int default_opts(int type, int dim, cufinufft_opts *opts)
     return 0;
 }
 
-// Default bin sizes (SETUP_BINSIZE, src/cufinufft.cu:17-73).
+// Default bin sizes (SETUP_BINSIZE, src/cufinufft.cu:17-73).  Method 4
+// takes the reference's 3D block defaults (output blocks 8^3, bins 4^3,
+// src/cufinufft.cu:53-66); 1D / 2D blocks, which the reference does not
+// have, default to the SM bin sizes.
 static void default_binsizes(int dim, cufinufft_opts &o)
 {
+    if (o.gpu_method == 4) {
+        const int ob = dim == 3 ? 8 : (dim == 2 ? 32 : 1024);
+        const int bs = dim == 3 ? 4 : ob;
+        if (o.gpu_obinsizex < 0) o.gpu_obinsizex = ob;
+        if (o.gpu_obinsizey < 0) o.gpu_obinsizey = dim > 1 ? ob : 1;
+        if (o.gpu_obinsizez < 0) o.gpu_obinsizez = dim > 2 ? ob : 1;
+        if (o.gpu_binsizex < 0) o.gpu_binsizex = bs;
+        if (o.gpu_binsizey < 0) o.gpu_binsizey = dim > 1 ? bs : 1;
+        if (o.gpu_binsizez < 0) o.gpu_binsizez = dim > 2 ? bs : 1;
+        if (dim < 3) o.gpu_obinsizez = o.gpu_binsizez = 1;
+        if (dim < 2) o.gpu_obinsizey = o.gpu_binsizey = 1;
+        return;
+    }
     if (dim == 1) {
         if (o.gpu_binsizex < 0) o.gpu_binsizex = 1024;
         o.gpu_binsizey = o.gpu_binsizez = 1;
@@ -528,7 +562,10 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
     for (int d = 0; d < dim; ++d)
         if (ms[d] < 1) return kErrNdataNotValid;
     if (ntransf_ < 1) return kErrNdataNotValid;
-    if (opts.gpu_method < 0 || opts.gpu_method > 2) return 2;
+    // methods: 0 auto, 1 GM(-sort), 2 SM, 4 owner-computes blocks (type 1);
+    // the reference's method 3 (Paul) is not built
+    if (opts.gpu_method < 0 || opts.gpu_method > 4 || opts.gpu_method == 3) return 2;
+    if (opts.gpu_method == 4 && type != 1) return 2;
     if (opts.gpu_maxsubprobsize < 1) return 1;
     if (opts.gpu_sort < 0 || opts.gpu_sort > 2) return 1;
     const int ier = setup_spreader<T>(sp, tol, (T)opts.upsampfac, opts.gpu_kerevalmeth);
@@ -560,18 +597,29 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
     }
 
     // Method: 0 picks SM when the tile fits the LDS, GM-sort otherwise.  The
-    // spread tile accumulates in FP64 whatever T is; the interp tile holds T.
+    // spread tile accumulates in FP64 (FP64 plans) or fixed point (FP32
+    // plans); the interp tile holds T.
+    fxwide = fx_wide(opts.gpu_maxsubprobsize, (double)tol);
     const int bsv[3] = {opts.gpu_binsizex, opts.gpu_binsizey, opts.gpu_binsizez};
     const size_t tile_bytes =
-        type == 1 ? spread_lds_bytes(dim, sp.ns, bsv, sizeof(T), opts.gpu_maxsubprobsize)
+        type == 1 ? spread_lds_bytes(dim, sp.ns, bsv, sizeof(T), opts.gpu_maxsubprobsize, fxwide)
                   : sm_tile_bytes(dim, sp.ns, bsv, sizeof(T));
     method = opts.gpu_method;
     if (method == 0) method = tile_bytes <= kMaxLdsBytes ? 2 : 1;
     if (method == 2 && tile_bytes > kMaxLdsBytes) return kErrLds;
+    const int obv[3] = {opts.gpu_obinsizex, opts.gpu_obinsizey, opts.gpu_obinsizez};
+    if (method == 4) {
+        // blocks must tile the fine grid and be whole numbers of bins, as
+        // in the reference (spread3d_wrapper.cu:410-441)
+        for (int d = 0; d < dim; ++d)
+            if (obv[d] < 1 || nf[d] % obv[d] != 0 || obv[d] % bsv[d] != 0) return 1;
+        if (blk_lds_bytes(dim, obv, sizeof(T), opts.gpu_maxsubprobsize, fxwide) > kMaxLdsBytes)
+            return kErrLds;
+    }
 
     for (int d = 0; d < 3; ++d) {
         geom.nf[d] = nf[d];
-        geom.bs[d] = d < dim ? bsv[d] : 1;
+        geom.bs[d] = d < dim ? (method == 4 ? obv[d] : bsv[d]) : 1;
         if (d >= dim) {
             geom.nb[d] = 1;
         } else if (sizeof(T) == 4) {
@@ -621,14 +669,59 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
         return kErrHip;
     const long long nparts = (nbins + kScanChunk - 1) / kScanChunk;
     if (hipMalloc(&sb.scan_part, sizeof(int2) * (size_t)nparts) != hipSuccess) return kErrHip;
-    if (opts.gpu_sort == 2) {  // fine (cell in bin) sort
-        const long long nkeys = nbins * (long long)geom.bs[0] * geom.bs[1] * geom.bs[2];
+    if (opts.gpu_sort == 2 || method == 4) {  // keyed sorts: (bin, cell) / block
+        const long long nkeys =
+            method == 4 ? nbins : nbins * (long long)geom.bs[0] * geom.bs[1] * geom.bs[2];
         if (nkeys > 2147483647LL) return kErrNdataNotValid;
         const long long kparts = (nkeys + kScanChunk - 1) / kScanChunk;
         if (dmalloc(&sb.kcount, (size_t)nkeys) || dmalloc(&sb.kstart, (size_t)nkeys) ||
             hipMalloc(&sb.kscan_part, sizeof(int2) * (size_t)kparts) != hipSuccess)
             return kErrHip;
     }
+    return 0;
+}
+
+// Method 4 setpts: count the copies of every point per block (one copy per
+// block its support touches), read the total back (the copy arrays are sized
+// by it; the reference's setpts also synchronises), then scatter.
+template <typename T>
+int Plan<T>::setpts_blocks()
+{
+    const int maxsub = opts.gpu_maxsubprobsize;
+    const int obv[3] = {geom.bs[0], geom.bs[1], geom.bs[2]};
+    if ((long long)M * blk_max_copies(dim, sp.ns, obv) > 2147483647LL) return kErrNdataNotValid;
+    tbegin();
+    int e = run_blk_count<T>(dim, geom, sp.ns, M, kx, ky, kz, sb, L);
+    if (e) return e;
+    int ncopies = 0;
+    if (hipMemcpyAsync(&ncopies, sb.counters, sizeof(int), hipMemcpyDeviceToHost, L.stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(L.stream) != hipSuccess)
+        return kErrHip;
+    if (ncopies > cap_copies) {
+        dfree(sb.idx);
+        dfree(sb.pos);
+        if (dmalloc(&sb.idx, (size_t)std::max(ncopies, 1)) ||
+            dmalloc(&sb.pos, (size_t)std::max(ncopies, 1) * dim)) {
+            cap_copies = -1;
+            return kErrHip;
+        }
+        cap_copies = ncopies;
+        sb.pos_stride = std::max(ncopies, 1);
+    }
+    sub_bound = nbins + (ncopies + (long long)maxsub - 1) / maxsub;
+    if (sub_bound > cap_sub) {
+        dfree(sb.s2b);
+        if (dmalloc(&sb.s2b, (size_t)sub_bound)) {
+            cap_sub = -1;
+            return kErrHip;
+        }
+        cap_sub = sub_bound;
+    }
+    e = run_blk_scatter<T>(dim, geom, sp.ns, M, ncopies, kx, ky, kz, maxsub, sb, L);
+    tend(kStageSort);
+    if (e) return e;
+    pts_set = true;
     return 0;
 }
 
@@ -642,7 +735,7 @@ int Plan<T>::setpts(int M_, const T *x, const T *y, const T *z)
     kx = x;
     ky = dim > 1 ? y : nullptr;
     kz = dim > 2 ? z : nullptr;
-    if (M > cap_M) {
+    if (M > cap_M && method != 4) {
         dfree(sb.rank);
         dfree(sb.binof);
         dfree(sb.idx);
@@ -656,6 +749,7 @@ int Plan<T>::setpts(int M_, const T *x, const T *y, const T *z)
         sb.pos_stride = M;
     }
     const int maxsub = opts.gpu_maxsubprobsize;
+    if (method == 4) return setpts_blocks();
     sub_bound = nbins + (M + (long long)maxsub - 1) / maxsub;
     const bool need_sub = method == 2;
     if (need_sub && sub_bound > cap_sub) {
@@ -910,7 +1004,8 @@ struct Group {
     {
         int e = 0;
         const Plan<T> &p0 = *plans[0];
-        const KerParams<T> kp = p0.ker_params();
+        KerParams<T> kp = p0.ker_params();
+        for (const auto *p : plans) kp.fxwide |= p->fxwide;  // the widest member's tile
         const int n = (int)plans.size();
         std::vector<hipStream_t> saved(n);
         for (int i = 0; i < n; ++i) {

@@ -21,11 +21,13 @@ inline int grid_for_points(int M, int ncu)
 // Persistent grid for a subproblem kernel: as many workgroups as fit on the
 // device at the tile's LDS footprint, never more than there are jobs, rounded
 // to whole XCD groups.  The LDS attribute and occupancy are cached per kernel
-// and LDS size, so launches after the first make no runtime queries (which
-// keeps them legal inside a stream capture).
-template <typename K>
-int sm_grid(K kern, size_t lds, long long jobs, int ncu, int *grid)
+// (the template is instantiated once per kernel, so kernels of one signature
+// keep separate caches) and LDS size, so launches after the first make no
+// runtime queries (which keeps them legal inside a stream capture).
+template <auto Kern>
+int sm_grid(size_t lds, long long jobs, int ncu, int *grid, int threads = 256)
 {
+    const auto kern = Kern;
     static thread_local size_t attr_lds = 64 * 1024, occ_lds = 0;
     static thread_local int occ_cached = 0;
     if (lds > attr_lds) {
@@ -36,7 +38,7 @@ int sm_grid(K kern, size_t lds, long long jobs, int ncu, int *grid)
     }
     int occ = occ_cached;
     if (lds != occ_lds || occ < 1) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 256, lds) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, threads, lds) != hipSuccess ||
             occ < 1)
             return kErrLds;
         occ_cached = occ;
@@ -59,11 +61,20 @@ int spread_ns(int method, int M, const SortBufs &sb, const cpx<T> *c, cpx<T> *fw
     if (method == 1) {
         k_spread_gm<T, DIM, NS><<<grid_for_points(M, L.ncu), 256, 0, L.stream>>>(
             M, sorted_pts(sb), c, fw, g, kp, ntr, nftot);
-    } else {
-        const size_t lds = spread_lds_bytes(DIM, NS, g.bs, sizeof(T), maxsub);
+    } else if (method == 4) {
+        const size_t lds = blk_lds_bytes(DIM, g.bs, sizeof(T), maxsub, kp.fxwide);
         if (lds > kMaxLdsBytes) return kErrLds;
         int grid = 0;
-        int e = sm_grid(k_spread_sm<T, DIM, NS>, lds, sub_bound * ntr, L.ncu, &grid);
+        int e = sm_grid<k_spread_blk<T, DIM, NS>>(lds, sub_bound * ntr, L.ncu, &grid);
+        if (e) return e;
+        SubprobArgs sa{sb.s2b, sb.substart, sb.binstart, sb.binsize, sb.counters, maxsub};
+        k_spread_blk<T, DIM, NS><<<grid, 256, lds, L.stream>>>(M, sorted_pts(sb), c, fw, g, kp,
+                                                               sa, ntr, nftot);
+    } else {
+        const size_t lds = spread_lds_bytes(DIM, NS, g.bs, sizeof(T), maxsub, kp.fxwide);
+        if (lds > kMaxLdsBytes) return kErrLds;
+        int grid = 0;
+        int e = sm_grid<k_spread_sm<T, DIM, NS>>(lds, sub_bound * ntr, L.ncu, &grid);
         if (e) return e;
         SubprobArgs sa{sb.s2b, sb.substart, sb.binstart, sb.binsize, sb.counters, maxsub};
         k_spread_sm<T, DIM, NS><<<grid, 256, lds, L.stream>>>(M, sorted_pts(sb), c, fw, g, kp, sa,
@@ -84,11 +95,18 @@ int interp_ns(int method, int M, const SortBufs &sb, cpx<T> *c, const cpx<T> *fw
         const size_t lds = sm_tile_bytes(DIM, NS, g.bs, sizeof(T));
         if (lds > kMaxLdsBytes) return kErrLds;
         int grid = 0;
-        int e = sm_grid(k_interp_sm<T, DIM, NS>, lds, sub_bound * ntr, L.ncu, &grid);
-        if (e) return e;
         SubprobArgs sa{sb.s2b, sb.substart, sb.binstart, sb.binsize, sb.counters, maxsub};
-        k_interp_sm<T, DIM, NS><<<grid, 256, lds, L.stream>>>(M, sorted_pts(sb), c, fw, g, kp, sa,
-                                                              ntr, nftot);
+        if (maxsub >= 512) {
+            int e = sm_grid<k_interp_sm<T, DIM, NS, 512>>(lds, sub_bound * ntr, L.ncu, &grid, 512);
+            if (e) return e;
+            k_interp_sm<T, DIM, NS, 512><<<grid, 512, lds, L.stream>>>(M, sorted_pts(sb), c, fw, g,
+                                                                       kp, sa, ntr, nftot);
+        } else {
+            int e = sm_grid<k_interp_sm<T, DIM, NS, 256>>(lds, sub_bound * ntr, L.ncu, &grid);
+            if (e) return e;
+            k_interp_sm<T, DIM, NS, 256><<<grid, 256, lds, L.stream>>>(M, sorted_pts(sb), c, fw, g,
+                                                                       kp, sa, ntr, nftot);
+        }
     }
     return hipGetLastError() == hipSuccess ? 0 : kErrHip;
 }
@@ -99,17 +117,17 @@ int group_ns(int type, const GroupItem<T> *items, int n, const long long *off,
              const Launch &L)
 {
     if (type == 1) {
-        const size_t lds = spread_lds_bytes(DIM, NS, bs, sizeof(T), maxsub);
+        const size_t lds = spread_lds_bytes(DIM, NS, bs, sizeof(T), maxsub, kp.fxwide);
         if (lds > kMaxLdsBytes) return kErrLds;
         int grid = 0;
-        int e = sm_grid(k_spread_sm_group<T, DIM, NS>, lds, jobs_bound, L.ncu, &grid);
+        int e = sm_grid<k_spread_sm_group<T, DIM, NS>>(lds, jobs_bound, L.ncu, &grid);
         if (e) return e;
         k_spread_sm_group<T, DIM, NS><<<grid, 256, lds, L.stream>>>(items, n, off, kp);
     } else {
         const size_t lds = sm_tile_bytes(DIM, NS, bs, sizeof(T));
         if (lds > kMaxLdsBytes) return kErrLds;
         int grid = 0;
-        int e = sm_grid(k_interp_sm_group<T, DIM, NS>, lds, jobs_bound, L.ncu, &grid);
+        int e = sm_grid<k_interp_sm_group<T, DIM, NS>>(lds, jobs_bound, L.ncu, &grid);
         if (e) return e;
         k_interp_sm_group<T, DIM, NS><<<grid, 256, lds, L.stream>>>(items, n, off, kp);
     }

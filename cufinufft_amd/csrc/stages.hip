@@ -30,10 +30,20 @@ size_t sm_tile_bytes(int dim, int ns, const int *bs, size_t elem)
     return 2 * cells * elem;
 }
 
-size_t spread_lds_bytes(int dim, int ns, const int *bs, size_t elem, int maxsub)
+size_t blk_lds_bytes(int dim, const int *ob, size_t elem, int maxsub, int fxwide)
+{
+    size_t cells = (size_t)ob[0];
+    if (dim > 1) cells *= (size_t)ob[1];
+    if (dim > 2) cells *= (size_t)ob[2];
+    (void)maxsub;
+    const int acc = elem == 4 ? (fxwide ? 8 : 4) : 8;
+    return blk_lds(cells, acc);
+}
+
+size_t spread_lds_bytes(int dim, int ns, const int *bs, size_t elem, int maxsub, int fxwide)
 {
     const size_t cells = sm_tile_bytes(dim, ns, bs, 1) / 2;
-    return elem == 4 ? fx_spread_lds(cells, maxsub) : 2 * cells * sizeof(double);
+    return elem == 4 ? fx_spread_lds(cells, maxsub, fxwide) : 2 * cells * sizeof(double);
 }
 
 template <typename T>
@@ -129,6 +139,67 @@ int run_fine_sort(int dim, const BinGeom &g, long long nbins, int M, const T *x,
     }
     if (need_subprob)
         k_map_subprob<<<grid_1d(nbins, L.ncu), 256, 0, L.stream>>>(nbins, sb.substart, sb.s2b);
+    return ok_or_err();
+}
+
+// Method 4, first half of setpts: count the copies of every point per
+// block and scan the counts; the total lands in counters[0] for the host to
+// size the copy arrays.
+template <typename T>
+int run_blk_count(int dim, const BinGeom &g, int ns, int M, const T *x, const T *y, const T *z,
+                  SortBufs &sb, const Launch &L)
+{
+    const long long nkeys = (long long)g.nb[0] * g.nb[1] * g.nb[2];
+    k_fill_int<<<grid_1d(nkeys, L.ncu, 64), 256, 0, L.stream>>>(nkeys, 0, sb.kcount);
+    if (M > 0) {
+        const int gp = grid_1d(M, L.ncu);
+        if (dim == 1)
+            k_blk_count<T, 1><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, ns, sb.kcount);
+        else if (dim == 2)
+            k_blk_count<T, 2><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, ns, sb.kcount);
+        else
+            k_blk_count<T, 3><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, ns, sb.kcount);
+    }
+    const long long kparts = (nkeys + kScanChunk - 1) / kScanChunk;
+    int2 *kp = static_cast<int2 *>(sb.kscan_part);
+    k_scan_reduce<<<(unsigned)kparts, kScanThreads, 0, L.stream>>>(sb.kcount, nkeys, 1, kp);
+    k_scan_parts<<<1, kScanThreads, 0, L.stream>>>(kp, (int)kparts, sb.counters, sb.counters + 1);
+    k_scan_apply<<<(unsigned)kparts, kScanThreads, 0, L.stream>>>(sb.kcount, nkeys, 1, kp,
+                                                                  sb.kstart, nullptr);
+    return ok_or_err();
+}
+
+// Method 4, second half: scatter the ncopies copies (index and shifted
+// positions), derive per-block sizes / starts and the subproblem map.
+template <typename T>
+int run_blk_scatter(int dim, const BinGeom &g, int ns, int M, int ncopies, const T *x,
+                    const T *y, const T *z, int maxsub, SortBufs &sb, const Launch &L)
+{
+    const long long nblocks = (long long)g.nb[0] * g.nb[1] * g.nb[2];
+    if (M > 0) {
+        const int gp = grid_1d(M, L.ncu);
+        const long long st = sb.pos_stride;
+        if (dim == 1)
+            k_blk_scatter<T, 1><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, ns, sb.kstart, sb.kcount,
+                                                          sb.idx, sb.pos, st);
+        else if (dim == 2)
+            k_blk_scatter<T, 2><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, ns, sb.kstart, sb.kcount,
+                                                          sb.idx, sb.pos, st);
+        else
+            k_blk_scatter<T, 3><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, ns, sb.kstart, sb.kcount,
+                                                          sb.idx, sb.pos, st);
+    }
+    k_bins_from_keys<<<grid_1d(nblocks, L.ncu), 256, 0, L.stream>>>(nblocks, 1, ncopies,
+                                                                    sb.kstart, sb.binsize);
+    const long long nparts = (nblocks + kScanChunk - 1) / kScanChunk;
+    int2 *part = static_cast<int2 *>(sb.scan_part);
+    k_scan_reduce<<<(unsigned)nparts, kScanThreads, 0, L.stream>>>(sb.binsize, nblocks, maxsub,
+                                                                   part);
+    k_scan_parts<<<1, kScanThreads, 0, L.stream>>>(part, (int)nparts, sb.counters,
+                                                   sb.substart + nblocks);
+    k_scan_apply<<<(unsigned)nparts, kScanThreads, 0, L.stream>>>(sb.binsize, nblocks, maxsub,
+                                                                  part, sb.binstart, sb.substart);
+    k_map_subprob<<<grid_1d(nblocks, L.ncu), 256, 0, L.stream>>>(nblocks, sb.substart, sb.s2b);
     return ok_or_err();
 }
 
@@ -242,6 +313,10 @@ int run_zero(cpx<T> *p, long long n, const Launch &L)
                                   const T *, int, bool, SortBufs &, const Launch &);         \
     template int run_natural_order<T>(int, const BinGeom &, int, const T *, const T *,       \
                                       const T *, SortBufs &, const Launch &);                \
+    template int run_blk_count<T>(int, const BinGeom &, int, int, const T *, const T *,      \
+                                  const T *, SortBufs &, const Launch &);                    \
+    template int run_blk_scatter<T>(int, const BinGeom &, int, int, int, const T *,          \
+                                    const T *, const T *, int, SortBufs &, const Launch &);  \
     template int run_spread<T>(int, int, int, int, const SortBufs &, const cpx<T> *,         \
                                cpx<T> *, const BinGeom &, const KerParams<T> &, int,         \
                                long long, int, long long, const Launch &);                   \

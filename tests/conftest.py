@@ -26,3 +26,12 @@ def rel_l2(a, b):
     a = np.asarray(a).ravel()
     b = np.asarray(b).ravel()
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def scale_err(got, ref):
+    """|s - 1| for the least-squares scale s = <ref, got> / <ref, ref>: a
+    systematic scaling of the whole result shows in full, random rounding
+    averages out over the cells."""
+    g = np.asarray(got).ravel().astype(np.complex128)
+    r = np.asarray(ref).ravel().astype(np.complex128)
+    return float(abs(np.vdot(r, g) / np.vdot(r, r) - 1.0))
