@@ -171,8 +171,10 @@ def measured_traffic(workload, kernel_prefix, opts):
             if not line.startswith("TRAFFIC "):
                 continue
             rec = json.loads(line[len("TRAFFIC "):])
+            kern = rec.get("kernel", "")
             if (rec.get("workload") == workload and rec.get("opts") == opts
-                    and rec.get("kernel", "").startswith(kernel_prefix)):
+                    and (kern.startswith(kernel_prefix + ">")
+                         or kern.startswith(kernel_prefix + ","))):
                 return float(rec["bytes"])
     return None
 
@@ -516,9 +518,10 @@ def main():
     nbytes = algorithmic_bytes(w, best["info"]["nf"], "interp" if w["type"] == 2 else "spread")
     achieved = nbytes / (kern_ms * 1e-3) / 1e9
     kname = ("k_interp_" if w["type"] == 2 else "k_spread_") + \
-        ("gm" if best["info"]["method"] == 1 else "sm")
+        {1: "gm", 2: "sm", 4: "blk"}[best["info"]["method"]]
+    # template arguments <T, DIM, NS[, threads]>: match up to NS
     ktmpl = f"void cfa::{kname}<{'float' if dt == np.float32 else 'double'}, {w['dim']}, " \
-            f"{best['info']['ns']}>"
+            f"{best['info']['ns']}"
     traffic = measured_traffic(a.workload, ktmpl, best_opts)
     res = {
         "metric": "NU pts/s per execute at rel-l2 <= tol: tuned/batched vs plain, 1 MI355X",

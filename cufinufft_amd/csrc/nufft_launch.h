@@ -54,8 +54,8 @@ template <typename T>
 int run_natural_order(int dim, const BinGeom &g, int M, const T *x, const T *y, const T *z,
                       SortBufs &sb, const Launch &L);
 // Method 4 (owner-computes blocks, g.bs = block size): count and scan the
-// point copies per (block, cell) key (total copies -> counters[0]), then
-// scatter them and build the per-block subproblems.
+// point copies per block (total copies -> counters[0]), then scatter them and
+// build the per-block subproblems.
 template <typename T>
 int run_blk_count(int dim, const BinGeom &g, int ns, int M, const T *x, const T *y, const T *z,
                   SortBufs &sb, const Launch &L);
@@ -76,8 +76,8 @@ int run_interp(int method, int dim, int ns, int M, const SortBufs &sb, cpx<T> *c
 
 // Bytes of LDS the SM kernels need for a tile (re and im of every cell).
 size_t sm_tile_bytes(int dim, int ns, const int *bs, size_t elem);
-// LDS bytes of the SM spread: 32-bit fixed-point tile plus staged strengths
-// for single precision (elem 4), FP64 tile for double precision.
+// LDS bytes of the SM spread: 32- or 64-bit fixed-point tile plus staged
+// strengths for single precision (elem 4), FP64 tile for double precision.
 size_t spread_lds_bytes(int dim, int ns, const int *bs, size_t elem, int maxsub, int fxwide);
 // LDS bytes of a method-4 (owner-computes block) spread job over blocks ob.
 size_t blk_lds_bytes(int dim, const int *ob, size_t elem, int maxsub, int fxwide);
