@@ -1378,9 +1378,12 @@ __device__ __forceinline__ void interp_sm_job(const SubprobJob &J, int M, const 
     __syncthreads();
 }
 
-// BT threads per workgroup: 512 (8 waves over one LDS tile) doubles the
-// waves per CU at the same LDS footprint when subproblems hold >= 512 points;
-// the FP64 interp is latency-bound at 2 tiles per CU otherwise.
+// BT threads per workgroup: 512 or 1024 (8 or 16 waves over one LDS tile)
+// multiply the waves per CU at the same LDS footprint when subproblems hold
+// that many points.  The FP64 interp is bound by LDS latency at 1-2 tiles per
+// CU (measured: sharing taps between points that read the same window, which
+// cuts LDS reads ~40% but needs ~200 VGPRs, ran slower), so waves that hide
+// the latency pay more than fewer reads.
 template <typename T, int DIM, int NS, int BT>
 __global__ void __launch_bounds__(BT)
 k_interp_sm(int M, SortedPts sp, cpx<T> *__restrict__ c, const cpx<T> *__restrict__ fw,

@@ -96,7 +96,13 @@ int interp_ns(int method, int M, const SortBufs &sb, cpx<T> *c, const cpx<T> *fw
         if (lds > kMaxLdsBytes) return kErrLds;
         int grid = 0;
         SubprobArgs sa{sb.s2b, sb.substart, sb.binstart, sb.binsize, sb.counters, maxsub};
-        if (maxsub >= 512) {
+        if (maxsub >= 1024) {
+            int e = sm_grid<k_interp_sm<T, DIM, NS, 1024>>(lds, sub_bound * ntr, L.ncu, &grid,
+                                                           1024);
+            if (e) return e;
+            k_interp_sm<T, DIM, NS, 1024><<<grid, 1024, lds, L.stream>>>(M, sorted_pts(sb), c, fw,
+                                                                         g, kp, sa, ntr, nftot);
+        } else if (maxsub >= 512) {
             int e = sm_grid<k_interp_sm<T, DIM, NS, 512>>(lds, sub_bound * ntr, L.ncu, &grid, 512);
             if (e) return e;
             k_interp_sm<T, DIM, NS, 512><<<grid, 512, lds, L.stream>>>(M, sorted_pts(sb), c, fw, g,
