@@ -485,7 +485,8 @@ def main():
             log("[bench] --kernel-only needs --opts")
             return 2
         r = run_plan(w, dev, dpts, din, json.loads(a.opts), a.steps, a.warmup, maxbatch)
-        log(f"[bench] kernel-only: {r['t_exec'] * 1e3:.3f} ms/exec, stages {r['stages']}")
+        log(f"[bench] kernel-only: {r['t_exec'] * 1e3:.3f} ms/exec, setpts "
+            f"{r['t_setpts'] * 1e3:.3f} ms, stages {r['stages']}")
         return 0
     log("[bench] plain plan (reference default options)")
     plain = run_plan(w, dev, dpts, din, {}, a.steps, a.warmup, maxbatch)

@@ -222,25 +222,10 @@ __device__ __forceinline__ void ladd(double *p, double v)
 }
 
 // ===========================================================================
-// setpts: counting bin sort
+// setpts: bin sort (keys here; the radix sort itself is in sort_radix.hip)
 // ===========================================================================
 
-// Per point: bin index and in-bin rank by atomic arrival
-// (CalcBinSize_noghost_{2,3}d, spreadinterp2d.cu:102-126).
-template <typename T, int DIM>
-__global__ void __launch_bounds__(256)
-k_bin_count(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
-            BinGeom g, int *__restrict__ binsize, int *__restrict__ rank,
-            int *__restrict__ binof)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
-        const int b = point_bin<T, DIM>(x, y, z, i, g);
-        binof[i] = b;
-        rank[i] = atomicAdd(&binsize[b], 1);
-    }
-}
-
-// Fine sort key: the bin (as above) then the fine-grid cell of the point
+// Fine sort key: the bin (point_bin) then the fine-grid cell of the point
 // inside its bin, key = bin * vol + local cell (vol = cells per bin).  Sorting
 // by this key orders each bin's points by cell, so neighbouring lanes of a
 // wave touch neighbouring LDS cells (or the same cell, which the LDS
@@ -257,25 +242,6 @@ __device__ __forceinline__ void bin_and_cell(double v, int bs, int nb, int &b, i
     lc = c < 0 ? 0 : (c >= bs ? bs - 1 : c);
 }
 
-template <typename T, int DIM>
-__global__ void __launch_bounds__(256)
-k_key_count(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
-            BinGeom g, int *__restrict__ keycount, int *__restrict__ rank,
-            int *__restrict__ keyof)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
-        int b0, c0, b1 = 0, c1 = 0, b2 = 0, c2 = 0;
-        bin_and_cell<T>(rescale_exact(x[i], g.nf[0]), g.bs[0], g.nb[0], b0, c0);
-        if (DIM > 1) bin_and_cell<T>(rescale_exact(y[i], g.nf[1]), g.bs[1], g.nb[1], b1, c1);
-        if (DIM > 2) bin_and_cell<T>(rescale_exact(z[i], g.nf[2]), g.bs[2], g.nb[2], b2, c2);
-        const int bin = b0 + g.nb[0] * (b1 + g.nb[1] * b2);
-        const int vol = g.bs[0] * g.bs[1] * g.bs[2];
-        const int key = bin * vol + c0 + g.bs[0] * (c1 + g.bs[1] * c2);
-        keyof[i] = key;
-        rank[i] = atomicAdd(&keycount[key], 1);
-    }
-}
-
 // Per-bin size and start from the key starts: the bin's keys are contiguous.
 static __global__ void __launch_bounds__(256)
 k_bins_from_keys(long long nbins, int vol, int M, const int *__restrict__ keystart,
@@ -286,24 +252,6 @@ k_bins_from_keys(long long nbins, int vol, int M, const int *__restrict__ keysta
         const int s0 = keystart[b * vol];
         const int s1 = b + 1 < nbins ? keystart[(b + 1) * vol] : M;
         binsize[b] = s1 - s0;
-    }
-}
-
-// idxnupts[binstart[b] + rank] = i (CalcInvertofGlobalSortIdx, spreadinterp2d.cu:
-// 128-149), and the point's rescaled positions into the same sorted slot.
-template <typename T, int DIM>
-__global__ void __launch_bounds__(256)
-k_bin_scatter(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
-              BinGeom g, const int *__restrict__ binof, const int *__restrict__ rank,
-              const int *__restrict__ binstart, int *__restrict__ idx, double *__restrict__ pos,
-              long long stride)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
-        const int s = binstart[binof[i]] + rank[i];
-        idx[s] = i;
-        pos[s] = tap_pos(rescale_exact(x[i], g.nf[0]), g.nf[0]);
-        if (DIM > 1) pos[stride + s] = tap_pos(rescale_exact(y[i], g.nf[1]), g.nf[1]);
-        if (DIM > 2) pos[2 * stride + s] = tap_pos(rescale_exact(z[i], g.nf[2]), g.nf[2]);
     }
 }
 

@@ -22,8 +22,11 @@ struct SortBufs {
     int *binstart = nullptr;    // [nbins]
     int *substart = nullptr;    // [nbins + 1]
     int *s2b = nullptr;         // [nbins + ceil(M / maxsub)] upper bound
-    int *rank = nullptr;        // [M]
-    int *binof = nullptr;       // [M]
+    int *skey = nullptr;        // [M] sort keys (bin, or bin and cell)
+    int *sval = nullptr;        // [M] point indices fed to the sort
+    int *skey_out = nullptr;    // [M] sorted keys
+    void *sort_tmp = nullptr;   // radix sort scratch
+    size_t sort_tmp_bytes = 0;
     int *idx = nullptr;         // [M] caller index of each sorted point
     double *pos = nullptr;      // [dim][pos_stride] sorted rescaled positions
     long long pos_stride = 0;   // capacity in points of each pos plane
@@ -40,15 +43,15 @@ struct Launch {
     int ncu = 256;
 };
 
-// Bin sort of M points (counts, ranks, scans, scatter of indices and
-// positions into sorted order, subproblem map).
+// Bin sort of M points: stable radix sort of the bin keys (fine: bin and
+// fine-grid cell inside the bin), per-bin counts, scans, positions gathered
+// into sorted order, subproblem map.
 template <typename T>
-int run_bin_sort(int dim, const BinGeom &g, long long nbins, int M, const T *x, const T *y,
-                 const T *z, int maxsub, bool need_subprob, SortBufs &sb, const Launch &L);
-// The same with points ordered by fine-grid cell inside each bin.
-template <typename T>
-int run_fine_sort(int dim, const BinGeom &g, long long nbins, int M, const T *x, const T *y,
-                  const T *z, int maxsub, bool need_subprob, SortBufs &sb, const Launch &L);
+int run_radix_sort(int dim, const BinGeom &g, long long nbins, int M, const T *x, const T *y,
+                   const T *z, int maxsub, bool fine, bool need_subprob, SortBufs &sb,
+                   const Launch &L);
+// Scratch bytes of that sort for M points and nkeys distinct keys.
+size_t radix_tmp_bytes(int M, long long nkeys);
 // Natural order (gpu_sort = 0): identity index and positions.
 template <typename T>
 int run_natural_order(int dim, const BinGeom &g, int M, const T *x, const T *y, const T *z,

@@ -433,8 +433,11 @@ struct Plan {
         dfree(sb.binstart);
         dfree(sb.substart);
         dfree(sb.s2b);
-        dfree(sb.rank);
-        dfree(sb.binof);
+        dfree(sb.skey);
+        dfree(sb.sval);
+        dfree(sb.skey_out);
+        if (sb.sort_tmp) hipFree(sb.sort_tmp);
+        sb.sort_tmp = nullptr;
         dfree(sb.idx);
         dfree(sb.pos);
         dfree(sb.counters);
@@ -669,10 +672,11 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
         return kErrHip;
     const long long nparts = (nbins + kScanChunk - 1) / kScanChunk;
     if (hipMalloc(&sb.scan_part, sizeof(int2) * (size_t)nparts) != hipSuccess) return kErrHip;
-    if (opts.gpu_sort == 2 || method == 4) {  // keyed sorts: (bin, cell) / block
-        const long long nkeys =
-            method == 4 ? nbins : nbins * (long long)geom.bs[0] * geom.bs[1] * geom.bs[2];
-        if (nkeys > 2147483647LL) return kErrNdataNotValid;
+    if (opts.gpu_sort == 2 &&
+        nbins * (long long)geom.bs[0] * geom.bs[1] * geom.bs[2] > 2147483647LL)
+        return kErrNdataNotValid;  // (bin, cell) keys are int
+    if (method == 4) {  // copies counted per block
+        const long long nkeys = nbins;
         const long long kparts = (nkeys + kScanChunk - 1) / kScanChunk;
         if (dmalloc(&sb.kcount, (size_t)nkeys) || dmalloc(&sb.kstart, (size_t)nkeys) ||
             hipMalloc(&sb.kscan_part, sizeof(int2) * (size_t)kparts) != hipSuccess)
@@ -736,17 +740,30 @@ int Plan<T>::setpts(int M_, const T *x, const T *y, const T *z)
     ky = dim > 1 ? y : nullptr;
     kz = dim > 2 ? z : nullptr;
     if (M > cap_M && method != 4) {
-        dfree(sb.rank);
-        dfree(sb.binof);
+        dfree(sb.skey);
+        dfree(sb.sval);
+        dfree(sb.skey_out);
         dfree(sb.idx);
         dfree(sb.pos);
-        if (dmalloc(&sb.rank, M) || dmalloc(&sb.binof, M) || dmalloc(&sb.idx, M) ||
-            dmalloc(&sb.pos, (size_t)M * dim)) {
+        if (dmalloc(&sb.skey, M) || dmalloc(&sb.sval, M) || dmalloc(&sb.skey_out, M) ||
+            dmalloc(&sb.idx, M) || dmalloc(&sb.pos, (size_t)M * dim)) {
             cap_M = -1;
             return kErrHip;
         }
         cap_M = M;
         sb.pos_stride = M;
+    }
+    if (method != 4 && !(method == 1 && !opts.gpu_sort)) {
+        const long long nkeys =
+            opts.gpu_sort == 2 ? nbins * (long long)geom.bs[0] * geom.bs[1] * geom.bs[2] : nbins;
+        const size_t need = radix_tmp_bytes(std::max(M, 1), nkeys);
+        if (need > sb.sort_tmp_bytes) {
+            if (sb.sort_tmp) hipFree(sb.sort_tmp);
+            sb.sort_tmp = nullptr;
+            sb.sort_tmp_bytes = 0;
+            if (hipMalloc(&sb.sort_tmp, need) != hipSuccess) return kErrHip;
+            sb.sort_tmp_bytes = need;
+        }
     }
     const int maxsub = opts.gpu_maxsubprobsize;
     if (method == 4) return setpts_blocks();
@@ -764,10 +781,9 @@ int Plan<T>::setpts(int M_, const T *x, const T *y, const T *z)
     tbegin();
     if (method == 1 && !opts.gpu_sort) {
         e = run_natural_order<T>(dim, geom, M, kx, ky, kz, sb, L);
-    } else if (opts.gpu_sort == 2) {
-        e = run_fine_sort<T>(dim, geom, nbins, M, kx, ky, kz, maxsub, need_sub, sb, L);
     } else {
-        e = run_bin_sort<T>(dim, geom, nbins, M, kx, ky, kz, maxsub, need_sub, sb, L);
+        e = run_radix_sort<T>(dim, geom, nbins, M, kx, ky, kz, maxsub, opts.gpu_sort == 2,
+                              need_sub, sb, L);
     }
     tend(kStageSort);
     if (e) return e;

@@ -46,102 +46,6 @@ size_t spread_lds_bytes(int dim, int ns, const int *bs, size_t elem, int maxsub,
     return elem == 4 ? fx_spread_lds(cells, maxsub, fxwide) : 2 * cells * sizeof(double);
 }
 
-template <typename T>
-int run_bin_sort(int dim, const BinGeom &g, long long nbins, int M, const T *x, const T *y,
-                 const T *z, int maxsub, bool need_subprob, SortBufs &sb, const Launch &L)
-{
-    k_fill_int<<<grid_1d(nbins, L.ncu), 256, 0, L.stream>>>(nbins, 0, sb.binsize);
-    if (M > 0) {
-        const int gp = grid_1d(M, L.ncu);
-        if (dim == 1)
-            k_bin_count<T, 1><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.binsize, sb.rank,
-                                                        sb.binof);
-        else if (dim == 2)
-            k_bin_count<T, 2><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.binsize, sb.rank,
-                                                        sb.binof);
-        else
-            k_bin_count<T, 3><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.binsize, sb.rank,
-                                                        sb.binof);
-    }
-    const long long nparts = (nbins + kScanChunk - 1) / kScanChunk;
-    int2 *part = static_cast<int2 *>(sb.scan_part);
-    k_scan_reduce<<<(unsigned)nparts, kScanThreads, 0, L.stream>>>(sb.binsize, nbins, maxsub,
-                                                                   part);
-    k_scan_parts<<<1, kScanThreads, 0, L.stream>>>(part, (int)nparts, sb.counters,
-                                                   sb.substart + nbins);
-    k_scan_apply<<<(unsigned)nparts, kScanThreads, 0, L.stream>>>(sb.binsize, nbins, maxsub,
-                                                                  part, sb.binstart, sb.substart);
-    if (M > 0) {
-        const int gp = grid_1d(M, L.ncu);
-        const long long st = sb.pos_stride;
-        if (dim == 1)
-            k_bin_scatter<T, 1><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.binof, sb.rank,
-                                                          sb.binstart, sb.idx, sb.pos, st);
-        else if (dim == 2)
-            k_bin_scatter<T, 2><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.binof, sb.rank,
-                                                          sb.binstart, sb.idx, sb.pos, st);
-        else
-            k_bin_scatter<T, 3><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.binof, sb.rank,
-                                                          sb.binstart, sb.idx, sb.pos, st);
-    }
-    if (need_subprob)
-        k_map_subprob<<<grid_1d(nbins, L.ncu), 256, 0, L.stream>>>(nbins, sb.substart, sb.s2b);
-    return ok_or_err();
-}
-
-// Fine sort: counting sort on (bin, cell in bin) keys; the bin-level arrays
-// (sizes, starts, subproblems) are derived from the key starts, so they equal
-// those of the bin sort exactly.
-template <typename T>
-int run_fine_sort(int dim, const BinGeom &g, long long nbins, int M, const T *x, const T *y,
-                  const T *z, int maxsub, bool need_subprob, SortBufs &sb, const Launch &L)
-{
-    const int vol = g.bs[0] * g.bs[1] * g.bs[2];
-    const long long nkeys = nbins * vol;
-    k_fill_int<<<grid_1d(nkeys, L.ncu, 64), 256, 0, L.stream>>>(nkeys, 0, sb.kcount);
-    if (M > 0) {
-        const int gp = grid_1d(M, L.ncu);
-        if (dim == 1)
-            k_key_count<T, 1><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.kcount, sb.rank, sb.binof);
-        else if (dim == 2)
-            k_key_count<T, 2><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.kcount, sb.rank, sb.binof);
-        else
-            k_key_count<T, 3><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.kcount, sb.rank, sb.binof);
-    }
-    const long long kparts = (nkeys + kScanChunk - 1) / kScanChunk;
-    int2 *kp = static_cast<int2 *>(sb.kscan_part);
-    k_scan_reduce<<<(unsigned)kparts, kScanThreads, 0, L.stream>>>(sb.kcount, nkeys, 1, kp);
-    k_scan_parts<<<1, kScanThreads, 0, L.stream>>>(kp, (int)kparts, nullptr, nullptr);
-    k_scan_apply<<<(unsigned)kparts, kScanThreads, 0, L.stream>>>(sb.kcount, nkeys, 1, kp,
-                                                                  sb.kstart, nullptr);
-    k_bins_from_keys<<<grid_1d(nbins, L.ncu), 256, 0, L.stream>>>(nbins, vol, M, sb.kstart,
-                                                                  sb.binsize);
-    const long long nparts = (nbins + kScanChunk - 1) / kScanChunk;
-    int2 *part = static_cast<int2 *>(sb.scan_part);
-    k_scan_reduce<<<(unsigned)nparts, kScanThreads, 0, L.stream>>>(sb.binsize, nbins, maxsub,
-                                                                   part);
-    k_scan_parts<<<1, kScanThreads, 0, L.stream>>>(part, (int)nparts, sb.counters,
-                                                   sb.substart + nbins);
-    k_scan_apply<<<(unsigned)nparts, kScanThreads, 0, L.stream>>>(sb.binsize, nbins, maxsub,
-                                                                  part, sb.binstart, sb.substart);
-    if (M > 0) {
-        const int gp = grid_1d(M, L.ncu);
-        const long long st = sb.pos_stride;
-        if (dim == 1)
-            k_bin_scatter<T, 1><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.binof, sb.rank,
-                                                          sb.kstart, sb.idx, sb.pos, st);
-        else if (dim == 2)
-            k_bin_scatter<T, 2><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.binof, sb.rank,
-                                                          sb.kstart, sb.idx, sb.pos, st);
-        else
-            k_bin_scatter<T, 3><<<gp, 256, 0, L.stream>>>(M, x, y, z, g, sb.binof, sb.rank,
-                                                          sb.kstart, sb.idx, sb.pos, st);
-    }
-    if (need_subprob)
-        k_map_subprob<<<grid_1d(nbins, L.ncu), 256, 0, L.stream>>>(nbins, sb.substart, sb.s2b);
-    return ok_or_err();
-}
-
 // Method 4, first half of setpts: count the copies of every point per
 // block and scan the counts; the total lands in counters[0] for the host to
 // size the copy arrays.
@@ -307,10 +211,6 @@ int run_zero(cpx<T> *p, long long n, const Launch &L)
 }
 
 #define CFA_INST(T)                                                                          \
-    template int run_bin_sort<T>(int, const BinGeom &, long long, int, const T *, const T *, \
-                                 const T *, int, bool, SortBufs &, const Launch &);          \
-    template int run_fine_sort<T>(int, const BinGeom &, long long, int, const T *, const T *, \
-                                  const T *, int, bool, SortBufs &, const Launch &);         \
     template int run_natural_order<T>(int, const BinGeom &, int, const T *, const T *,       \
                                       const T *, SortBufs &, const Launch &);                \
     template int run_blk_count<T>(int, const BinGeom &, int, int, const T *, const T *,      \
