@@ -947,6 +947,14 @@ struct Group {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     Launch L;
+    // Branch streams for the per-member passes (zero fill, FFT, deconvolve /
+    // amplify): small members leave most CUs idle, so members run
+    // concurrently, round-robin over kBranches streams forked from and joined
+    // back to the group stream with events (parallel branches in a captured
+    // graph).  Four: the hardware queues a process gets by default.
+    static constexpr int kBranches = 4;
+    hipStream_t br[kBranches] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kBranches] = {};
 
     ~Group()
     {
@@ -955,6 +963,11 @@ struct Group {
         if (graph) hipGraphDestroy(graph);
         if (ev_in) hipEventDestroy(ev_in);
         if (ev_out) hipEventDestroy(ev_out);
+        if (ev_fork) hipEventDestroy(ev_fork);
+        for (int b = 0; b < kBranches; ++b) {
+            if (ev_join[b]) hipEventDestroy(ev_join[b]);
+            if (br[b]) hipStreamDestroy(br[b]);
+        }
         if (stream) hipStreamDestroy(stream);
         if (items) hipFree(items);
         if (off) hipFree(off);
@@ -988,8 +1001,13 @@ struct Group {
         if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return kErrHip;
         L.stream = stream;
         if (hipEventCreateWithFlags(&ev_in, hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&ev_out, hipEventDisableTiming) != hipSuccess)
+            hipEventCreateWithFlags(&ev_out, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming) != hipSuccess)
             return kErrHip;
+        for (int b = 0; b < kBranches; ++b)
+            if (hipStreamCreateWithFlags(&br[b], hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&ev_join[b], hipEventDisableTiming) != hipSuccess)
+                return kErrHip;
         const size_t isz = group_item_bytes(sizeof(T) == 8);
         std::vector<unsigned char> h(isz * n);
         for (int i = 0; i < n; ++i) {
@@ -1035,23 +1053,46 @@ struct Group {
                 return group_dim<T, 2>(type, ns, items, n, off, kp, bs, maxsub, jobs_bound, L);
             return group_dim<T, 3>(type, ns, items, n, off, kp, bs, maxsub, jobs_bound, L);
         };
+        // per-member passes on the branch streams, between a fork and a join
+        auto branches = [&](auto member) {
+            if (hipEventRecord(ev_fork, stream) != hipSuccess) return kErrHip;
+            for (int b = 0; b < kBranches; ++b)
+                if (hipStreamWaitEvent(br[b], ev_fork, 0) != hipSuccess) return kErrHip;
+            int err = 0;
+            for (int i = 0; i < n && !err; ++i) {
+                Launch Lb = L;
+                Lb.stream = br[i % kBranches];
+                plans[i]->L.stream = Lb.stream;  // the member's FFT follows its branch
+                err = member(i, Lb);
+            }
+            for (int b = 0; b < kBranches; ++b)
+                if (hipEventRecord(ev_join[b], br[b]) != hipSuccess ||
+                    hipStreamWaitEvent(stream, ev_join[b], 0) != hipSuccess)
+                    return kErrHip;
+            return err;
+        };
         if (type == 1) {
-            for (int i = 0; i < n && !e; ++i)
-                e = run_zero<T>(plans[i]->fw, plans[i]->nftot * plans[i]->ntransf, L);
+            e = branches([&](int i, const Launch &Lb) {
+                return run_zero<T>(plans[i]->fw, plans[i]->nftot * plans[i]->ntransf, Lb);
+            });
             if (!e) e = spread_or_interp();
-            for (int i = 0; i < n && !e; ++i) {
-                Plan<T> &p = *plans[i];
-                const T *kers[3] = {p.fwker[0], p.fwker[1], p.fwker[2]};
-                e = p.fft(p.fw, p.ntransf);
-                if (!e) e = run_deconvolve<T>(dim, p.ms, p.nf, p.fw, fk[i], kers, p.ntransf, L);
-            }
+            if (!e)
+                e = branches([&](int i, const Launch &Lb) {
+                    Plan<T> &p = *plans[i];
+                    const T *kers[3] = {p.fwker[0], p.fwker[1], p.fwker[2]};
+                    int err = p.fft(p.fw, p.ntransf);
+                    if (!err)
+                        err = run_deconvolve<T>(dim, p.ms, p.nf, p.fw, fk[i], kers, p.ntransf, Lb);
+                    return err;
+                });
         } else {
-            for (int i = 0; i < n && !e; ++i) {
+            e = branches([&](int i, const Launch &Lb) {
                 Plan<T> &p = *plans[i];
                 const T *kers[3] = {p.fwker[0], p.fwker[1], p.fwker[2]};
-                e = run_amplify<T>(dim, p.ms, p.nf, p.fw, fk[i], kers, p.ntransf, L);
-                if (!e) e = p.fft(p.fw, p.ntransf);
-            }
+                int err = run_amplify<T>(dim, p.ms, p.nf, p.fw, fk[i], kers, p.ntransf, Lb);
+                if (!err) err = p.fft(p.fw, p.ntransf);
+                return err;
+            });
             if (!e) e = spread_or_interp();
         }
         for (int i = 0; i < n; ++i) plans[i]->L.stream = saved[i];
