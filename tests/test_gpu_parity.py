@@ -233,6 +233,35 @@ def test_many_batched(torch_dev):
         assert rel_l2(cc[t * M:(t + 1) * M], ref) <= tol / 5
 
 
+@pytest.mark.parametrize("opts", [dict(gpu_method=2),
+                                  dict(gpu_method=2, gpu_binsizex=8, gpu_binsizey=8,
+                                       gpu_binsizez=8, gpu_maxsubprobsize=256,
+                                       gpu_kerevalmeth=1)],
+                         ids=["default", "tuned-shape"])
+def test_type1_fp64_3d_repeated(torch_dev, opts):
+    """FP64 3D type 1 on 1e6 clustered points (the large-workload path at a
+    size the exact sum finishes quickly): three setpts and four executes per
+    setpts on one plan, every output against the FP64 direct sum."""
+    import torch
+    from cufinufft_amd import cufinufft
+    nm, M, tol = (64, 64, 64), 1_000_000, 1e-9
+    _flush(f"fp64 3d repeated {opts}")
+    plan = cufinufft(1, nm[::-1], eps=tol, dtype=np.float64, **opts)
+    kidx = np.random.default_rng(12).choice(int(np.prod(nm)), 200, replace=False)
+    for seed in (31, 32, 31):
+        pts = G.clustered_pts(M, 3, np.float64, seed, ncl=4, sigma=0.2)
+        c = G.strengths(M, np.float64, seed=seed)
+        dpts = [G.to_dev(p, torch_dev) for p in pts]
+        plan.set_pts(*dpts[::-1])
+        dc = G.to_dev(c, torch_dev)
+        direct = O.dirft_type1(pts, c, list(nm), +1, kidx)
+        fk = torch.empty(nm[::-1], dtype=torch.complex128, device=torch_dev)
+        for _ in range(4):
+            plan.execute(dc, fk)
+            err = rel_l2(fk.cpu().numpy().ravel()[kidx], direct)
+            assert err < 4e-9, (seed, err)
+
+
 def test_setpts_twice(torch_dev):
     """Re-setpts on one plan (test/cufinufft2d1nupts_test.cu:145-186)."""
     import torch
