@@ -12,11 +12,11 @@ if ``CUFINUFFT_AMD_TUNE_CACHE`` names a JSON file, across processes.
     best, table = autotune.tune(1, (256, 256, 256), pts, eps=1e-5, dtype=np.float32)
     plan = cufinufft(1, (256, 256, 256), eps=1e-5, dtype=np.float32, **best)
 
-The candidate list follows the hardware: spread tiles accumulate in FP64 LDS
-(16 bytes per cell) for FP64 plans and in u32 fixed point (8 bytes per cell)
-for FP32 plans, interp tiles hold one complex per cell, all capped at the
-160 KiB LDS of a gfx950 workgroup; plans whose tile does not fit are skipped
-(makeplan reports it).
+The candidate list follows the hardware: spread tiles accumulate in 64-bit
+fixed point (16 bytes per cell) for FP64 plans and in 32- or 64-bit fixed
+point for FP32 plans, interp tiles hold one complex per cell, all capped at
+the 160 KiB LDS of a gfx950 workgroup; plans whose tile does not fit are
+skipped (makeplan reports it).
 """
 import json
 import os

@@ -690,51 +690,48 @@ __device__ __forceinline__ SubprobJob subprob_job(long long job, int total, cons
 // with global float atomics (the halo overlaps neighbouring bins).  The job
 // count comes from device memory, so setpts never copies it back.
 //
-// The tile accumulates in FP64 for both plan precisions: on gfx950 the LDS
-// ds_add_f64 sustains about 2.5 lane-ops per CU clock against 0.33 for
-// ds_add_f32 (measured), so the wider tile is 7.6x faster as well as more
-// accurate than FP32 accumulation.
 // Kernel values normalised by their peak: phi(t) / exp(beta) in (0, 1].
-template <int NS>
-__device__ __forceinline__ int taps_local_unit(double v, int o, int tw, const KerParams<float> &kp,
-                                               float *ker)
+template <typename T, int NS>
+__device__ __forceinline__ int taps_local_unit(double v, int o, int tw, const KerParams<T> &kp,
+                                               T *ker)
 {
     const int l0 = (int)ceil(v - 0.5 * NS) - o;
     const int l = l0 < 0 ? 0 : (l0 > tw - NS ? tw - NS : l0);
     if (kp.hdeg > 0 && l == l0) {
-        horner_taps<float, NS>((float)(2.0 * ((double)(o + l) - v) + (NS - 1)),
-                               kp.hc + (kp.hdeg + 1) * NS, kp.hdeg, ker);
+        horner_taps<T, NS>((T)(2.0 * ((double)(o + l) - v) + (NS - 1)),
+                           kp.hc + (kp.hdeg + 1) * NS, kp.hdeg, ker);
         return l;
     }
-    const float x1 = (float)((double)(o + l) - v);
+    const T x1 = (T)((double)(o + l) - v);
 #pragma unroll
     for (int i = 0; i < NS; ++i) {
-        const float t = x1 + (float)i;
-        const float a = fmaxf(1.0f - kp.c * t * t, 0.0f);
-        ker[i] = fabsf(t) < kp.hw ? expf(kp.beta * (sqrtf(a) - 1.0f)) : 0.0f;
+        const T t = x1 + (T)i;
+        const T a = fmax(T(1) - kp.c * t * t, T(0));
+        ker[i] = fabs(t) < kp.hw ? exp(kp.beta * (sqrt(a) - T(1))) : T(0);
     }
     return l;
 }
 
-// Sum of a float over the 256 threads of the workgroup (returned to all).
-__device__ __forceinline__ float block_sum256(float v, float *scratch)
+// Sum over the 256 threads of the workgroup (returned to all).
+template <typename T>
+__device__ __forceinline__ T block_sum256(T v, T *scratch)
 {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
     const int wid = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) scratch[wid] = v;
     __syncthreads();
-    const float s = scratch[0] + scratch[1] + scratch[2] + scratch[3];
+    const T s = scratch[0] + scratch[1] + scratch[2] + scratch[3];
     __syncthreads();
     return s;
 }
 
-// LDS bytes of the single-precision fixed-point spread: 32- or 64-bit re / im
-// per tile cell, the job's strengths staged as complex floats, and 4 floats
-// of reduction scratch.
-__host__ __device__ inline size_t fx_spread_lds(size_t cells, int maxsub, int wide)
+// LDS bytes of the fixed-point SM spread: 32- or 64-bit re / im per tile
+// cell, the job's strengths staged as complex T (elem = sizeof(T)), and 4
+// values of reduction scratch.
+__host__ __device__ inline size_t fx_spread_lds(size_t cells, int maxsub, int wide, int elem)
 {
-    return cells * (wide ? 16 : 8) + (size_t)maxsub * 8 + 16;
+    return cells * (wide ? 16 : 8) + (size_t)maxsub * 2 * elem + 4 * (size_t)elem;
 }
 
 // Fixed-point accumulator widths: headroom bits of the tile scale, the
@@ -750,65 +747,70 @@ template <> struct FxAcc<int> {
 template <> struct FxAcc<long long> {
     static constexpr int kBits = 62;
     __device__ static long long cvt(float v) { return __float2ll_rn(v); }
+    __device__ static long long cvt(double v) { return __double2ll_rn(v); }
     __device__ static void add(long long *p, long long v)
     {
         atomicAdd(reinterpret_cast<unsigned long long *>(p), (unsigned long long)v);
     }
 };
 
-// Single-precision SM spread job with fixed-point LDS accumulation (Acc =
-// int or long long).  On gfx950 ds_add_u32 sustains 8-14 lane-ops per CU
-// clock, ds_add_u64 4.9, against 2.5 for ds_add_f64 and 0.33 for ds_add_f32
-// (measured).  Exactness by construction: with the kernel normalised to
-// phi/exp(beta) <= 1 per axis, no tile cell can exceed
-// sum_p (|re c_p| + |im c_p|) =: A over the job's points, so the job scales
-// contributions by the power of two S = 2^kBits / A (rounded down), inside
-// the signed range; each contribution is rounded to nearest, an absolute
-// error of at most A * 2^-(kBits+1) per tap.  With 32 bits that error grows
-// with the points per job (A); the plan picks 64 bits when it would matter
-// at its tolerance.  The flush multiplies by exp(DIM beta) / S in FP64.
-template <typename Acc, int DIM, int NS>
+// SM spread job with fixed-point LDS accumulation, for both plan precisions
+// (T = float with Acc = int or long long, T = double with Acc = long long).
+// On gfx950 ds_add_u32 sustains 8-14 lane-ops per CU clock, ds_add_u64 4.9,
+// against 2.5 for ds_add_f64 and 0.33 for ds_add_f32 (measured).  Exactness
+// by construction: with the kernel normalised to phi/exp(beta) <= 1 per axis,
+// no tile cell can exceed sum_p (|re c_p| + |im c_p|) =: A over the job's
+// points, so the job scales contributions by the power of two
+// S = 2^kBits / A (rounded down), inside the signed range; each contribution
+// is rounded to nearest, an absolute error of at most A * 2^-(kBits+1) per
+// tap.  With 32 bits that error grows with the points per job (A); FP32 plans
+// pick 64 bits when it would matter at their tolerance, and FP64 plans always
+// use 64 bits (A * 2^-63 per tap: the rounding level of FP64 sums of size A).
+// The flush multiplies by exp(DIM beta) / S in FP64.
+template <typename T, typename Acc, int DIM, int NS>
 __device__ __forceinline__ void spread_sm_job_fx(const SubprobJob &J, int M, const SortedPts &sp,
-                                                 const cpx<float> *__restrict__ c,
-                                                 cpx<float> *__restrict__ fw, const BinGeom &g,
-                                                 const KerParams<float> &kp, long long nftot,
+                                                 const cpx<T> *__restrict__ c,
+                                                 cpx<T> *__restrict__ fw, const BinGeom &g,
+                                                 const KerParams<T> &kp, long long nftot,
                                                  unsigned char *smem, int maxsub)
 {
     const Tile<DIM, NS> tl(g, J.b);
     Acc *sre = reinterpret_cast<Acc *>(smem);
     Acc *sim = sre + tl.n;
-    cpx<float> *cst = reinterpret_cast<cpx<float> *>(sim + tl.n);  // 2*sizeof(Acc)*tl.n bytes in: aligned
-    float *red = reinterpret_cast<float *>(cst + maxsub);
+    cpx<T> *cst = reinterpret_cast<cpx<T> *>(sim + tl.n);  // 2*sizeof(Acc)*tl.n bytes in: aligned
+    T *red = reinterpret_cast<T *>(cst + maxsub);
     for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
         sre[q] = 0;
         sim[q] = 0;
     }
-    float part = 0.0f;
+    T part = 0;
     for (int q = threadIdx.x; q < J.np; q += blockDim.x) {
-        const cpx<float> cv = c[(long long)J.t * M + sp.idx[J.p0 + q]];
+        const cpx<T> cv = c[(long long)J.t * M + sp.idx[J.p0 + q]];
         cst[q] = cv;
-        part += fabsf(cv.x) + fabsf(cv.y);
+        part += fabs(cv.x) + fabs(cv.y);
     }
-    const float A = block_sum256(part, red);  // also orders the stores above
+    const T A = block_sum256(part, red);  // also orders the stores above
     int ex = 0;
-    frexpf(A, &ex);                            // A < 2^ex
-    const int sh = A > 0.0f ? FxAcc<Acc>::kBits - ex : 0;  // S = 2^sh, applied with ldexp
+    frexp(A, &ex);                        // A < 2^ex
+    const int sh = A > T(0) ? FxAcc<Acc>::kBits - ex : 0;  // S = 2^sh, applied with ldexp
     for (int q = threadIdx.x; q < J.np; q += blockDim.x) {
         const int i = J.p0 + q;
-        float k1[NS], k2[NS], k3[NS];
-        const int l1 = taps_local_unit<NS>(sp.pos[i], tl.o[0], tl.tw[0], kp, k1);
+        T k1[NS], k2[NS], k3[NS];
+        const int l1 = taps_local_unit<T, NS>(sp.pos[i], tl.o[0], tl.tw[0], kp, k1);
         int l2 = 0, l3 = 0;
-        if (DIM > 1) l2 = taps_local_unit<NS>(sp.pos[sp.stride + i], tl.o[1], tl.tw[1], kp, k2);
-        if (DIM > 2) l3 = taps_local_unit<NS>(sp.pos[2 * sp.stride + i], tl.o[2], tl.tw[2], kp, k3);
-        const cpx<float> cv = cst[q];
-        const float sr = ldexpf(cv.x, sh), si = ldexpf(cv.y, sh);
+        if (DIM > 1)
+            l2 = taps_local_unit<T, NS>(sp.pos[sp.stride + i], tl.o[1], tl.tw[1], kp, k2);
+        if (DIM > 2)
+            l3 = taps_local_unit<T, NS>(sp.pos[2 * sp.stride + i], tl.o[2], tl.tw[2], kp, k3);
+        const cpx<T> cv = cst[q];
+        const T sr = ldexp(cv.x, sh), si = ldexp(cv.y, sh);
 #pragma unroll
         for (int dz = 0; dz < (DIM > 2 ? NS : 1); ++dz) {
-            const float kz = DIM > 2 ? k3[dz] : 1.0f;
+            const T kz = DIM > 2 ? k3[dz] : T(1);
 #pragma unroll
             for (int dy = 0; dy < (DIM > 1 ? NS : 1); ++dy) {
-                const float kyz = (DIM > 1 ? k2[dy] : 1.0f) * kz;
-                const float cr = sr * kyz, ci = si * kyz;
+                const T kyz = (DIM > 1 ? k2[dy] : T(1)) * kz;
+                const T cr = sr * kyz, ci = si * kyz;
                 const int row = ((l3 + dz) * tl.tw[1] + (l2 + dy)) * tl.tw[0] + l1;
 #pragma unroll
                 for (int a = 0; a < NS; ++a) {
@@ -822,66 +824,13 @@ __device__ __forceinline__ void spread_sm_job_fx(const SubprobJob &J, int M, con
     // e^(DIM*beta) / S in FP64: in FP32 the rounding of DIM*beta (~50) alone
     // would scale the whole tile by up to 2e-6, and 2^-sh can leave FP32 range
     const double back = ldexp(exp((double)DIM * (double)kp.beta), -sh);
-    cpx<float> *f = fw + (long long)J.t * nftot;
+    cpx<T> *f = fw + (long long)J.t * nftot;
     for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
         const Acc ir = sre[q], ii = sim[q];
         if (ir != 0 || ii != 0) {
-            cpx<float> *p = f + tl.grid_offset(q, g);
-            gadd(&p->x, (float)((double)ir * back));
-            gadd(&p->y, (float)((double)ii * back));
-        }
-    }
-    __syncthreads();
-}
-
-template <typename T, int DIM, int NS>
-__device__ __forceinline__ void spread_sm_job(const SubprobJob &J, int M, const SortedPts &sp,
-                                              const cpx<T> *__restrict__ c,
-                                              cpx<T> *__restrict__ fw, const BinGeom &g,
-                                              const KerParams<T> &kp, long long nftot,
-                                              unsigned char *smem)
-{
-    const Tile<DIM, NS> tl(g, J.b);
-    double *sre = reinterpret_cast<double *>(smem);
-    double *sim = sre + tl.n;
-    for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
-        sre[q] = 0.0;
-        sim[q] = 0.0;
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < J.np; q += blockDim.x) {
-        const int i = J.p0 + q;
-        const int j = sp.idx[i];
-        T k1[NS], k2[NS], k3[NS];
-        const int l1 = taps_local<T, NS>(sp.pos[i], tl.o[0], tl.tw[0], kp, k1);
-        int l2 = 0, l3 = 0;
-        if (DIM > 1) l2 = taps_local<T, NS>(sp.pos[sp.stride + i], tl.o[1], tl.tw[1], kp, k2);
-        if (DIM > 2) l3 = taps_local<T, NS>(sp.pos[2 * sp.stride + i], tl.o[2], tl.tw[2], kp, k3);
-        const cpx<T> cv = c[(long long)J.t * M + j];
-#pragma unroll
-        for (int dz = 0; dz < (DIM > 2 ? NS : 1); ++dz) {
-            const T kz = DIM > 2 ? k3[dz] : (T)1;
-#pragma unroll
-            for (int dy = 0; dy < (DIM > 1 ? NS : 1); ++dy) {
-                const T kyz = (DIM > 1 ? k2[dy] : (T)1) * kz;
-                const T cr = cv.x * kyz, ci = cv.y * kyz;
-                const int row = ((l3 + dz) * tl.tw[1] + (l2 + dy)) * tl.tw[0] + l1;
-#pragma unroll
-                for (int a = 0; a < NS; ++a) {
-                    ladd(&sre[row + a], (double)(cr * k1[a]));
-                    ladd(&sim[row + a], (double)(ci * k1[a]));
-                }
-            }
-        }
-    }
-    __syncthreads();
-    cpx<T> *f = fw + (long long)J.t * nftot;
-    for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
-        const T vr = (T)sre[q], vi = (T)sim[q];
-        if (vr != (T)0 || vi != (T)0) {
             cpx<T> *p = f + tl.grid_offset(q, g);
-            gadd(&p->x, vr);
-            gadd(&p->y, vi);
+            gadd(&p->x, (T)((double)ir * back));
+            gadd(&p->y, (T)((double)ii * back));
         }
     }
     __syncthreads();
@@ -896,16 +845,12 @@ k_spread_sm(int M, SortedPts sp, const cpx<T> *__restrict__ c, cpx<T> *__restric
     const int total = sa.counters[1];
     const JobRange jr = xcd_jobs((long long)total * ntr);
     for (long long job = jr.begin; job < jr.end; job += jr.step) {
-        if constexpr (sizeof(T) == 4) {
-            if (kp.fxwide)
-                spread_sm_job_fx<long long, DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g,
-                                                     kp, nftot, smem, sa.maxsub);
-            else
-                spread_sm_job_fx<int, DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g, kp,
-                                               nftot, smem, sa.maxsub);
-        } else
-            spread_sm_job<T, DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g, kp, nftot,
-                                      smem);
+        if (sizeof(T) == 4 && !kp.fxwide)
+            spread_sm_job_fx<T, int, DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g, kp,
+                                              nftot, smem, sa.maxsub);
+        else
+            spread_sm_job_fx<T, long long, DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g,
+                                                    kp, nftot, smem, sa.maxsub);
     }
 }
 
@@ -964,15 +909,12 @@ k_spread_sm_group(const GroupItem<T> *__restrict__ items, int n,
         const int p = group_member(off, n, job);
         const GroupItem<T> &it = items[p];
         const SubprobJob J = subprob_job(job - off[p], it.sa.counters[1], it.sa);
-        if constexpr (sizeof(T) == 4) {
-            if (kp.fxwide)
-                spread_sm_job_fx<long long, DIM, NS>(J, it.M, it.sp, it.in, it.out, it.g, kp,
-                                                     it.nftot, smem, it.sa.maxsub);
-            else
-                spread_sm_job_fx<int, DIM, NS>(J, it.M, it.sp, it.in, it.out, it.g, kp, it.nftot,
-                                               smem, it.sa.maxsub);
-        } else
-            spread_sm_job<T, DIM, NS>(J, it.M, it.sp, it.in, it.out, it.g, kp, it.nftot, smem);
+        if (sizeof(T) == 4 && !kp.fxwide)
+            spread_sm_job_fx<T, int, DIM, NS>(J, it.M, it.sp, it.in, it.out, it.g, kp, it.nftot,
+                                              smem, it.sa.maxsub);
+        else
+            spread_sm_job_fx<T, long long, DIM, NS>(J, it.M, it.sp, it.in, it.out, it.g, kp,
+                                                    it.nftot, smem, it.sa.maxsub);
     }
 }
 

@@ -43,7 +43,7 @@ size_t blk_lds_bytes(int dim, const int *ob, size_t elem, int maxsub, int fxwide
 size_t spread_lds_bytes(int dim, int ns, const int *bs, size_t elem, int maxsub, int fxwide)
 {
     const size_t cells = sm_tile_bytes(dim, ns, bs, 1) / 2;
-    return elem == 4 ? fx_spread_lds(cells, maxsub, fxwide) : 2 * cells * sizeof(double);
+    return elem == 4 ? fx_spread_lds(cells, maxsub, fxwide, 4) : fx_spread_lds(cells, maxsub, 1, 8);
 }
 
 // Method 4, first half of setpts: count the copies of every point per
