@@ -22,8 +22,9 @@ type 2, s = isign):
   from one execute_grad of the sign -s type-2 plan.
 
 Gradients sum over the ntransf transforms that share the points.  Plans are
-cached per (type, modes, precision, eps, sign, ntransf); set_pts runs on every
-call.
+cached per (type, modes, precision, eps, sign, ntransf); set_pts runs when the
+points differ from the ones the plan last sorted (a forward and its backward
+on unchanged points sort once per plan).
 """
 import numpy as np
 import torch
@@ -48,9 +49,21 @@ def _np_real(t):
 
 
 def _set_pts(plan, points):
+    """set_pts unless this plan already holds these points: same storage
+    address, shape, strides and dtype, and the same autograd version counter
+    (bumped by every in-place write, shared by views).  The plan keeps a
+    reference to the tensor it last sorted, so that memory cannot be reused by
+    another tensor while the key is cached.  (Writes through ``.data`` bypass
+    the version counter and are not detected.)"""
+    key = (points.data_ptr(), tuple(points.shape), tuple(points.stride()), points.dtype,
+           points._version)
+    held = getattr(plan, "_pts_held", None)
+    if held is not None and held[1] == key:
+        return
     # the class takes numpy axis order (..., y, x) and reverses it
     rows = [points[d].contiguous() for d in range(points.shape[0])]
     plan.set_pts(*rows[::-1])
+    plan._pts_held = (points, key)
 
 
 def _cdtype(rdtype):

@@ -142,3 +142,26 @@ def test_type2_points_grad_finite_difference(torch_dev):
         assert abs(f1 - g[d, j]) <= 4 / 3 * abs(f1 - f2) * 1.5 + 1e-6 * scale, (d, j, f1, g[d, j])
         rich = (4 * f2 - f1) / 3
         assert abs(rich - g[d, j]) <= 1e-6 * scale, (d, j, rich, g[d, j])
+
+
+def test_points_cache_follows_inplace_writes(torch_dev):
+    """A plan re-sorts when the points tensor is written in place (autograd
+    version counter) and when a different tensor is passed; the forward on
+    updated points matches the dense FP64 model."""
+    import torch
+    from cufinufft_amd.torch_nufft import nufft_type2
+    n_modes = (24, 20)
+    pts = torch.tensor(spokes(16, 32, 1e-3, 3, np.float64), device=torch_dev)
+    rng = np.random.default_rng(4)
+    fk = torch.tensor(rng.standard_normal(n_modes[::-1]) + 1j * rng.standard_normal(n_modes[::-1]),
+                      device=torch_dev)
+    c1 = nufft_type2(pts, fk, isign=-1, eps=1e-10)
+    with torch.no_grad():
+        pts.mul_(0.9)  # in place: same storage, new version
+    c2 = nufft_type2(pts, fk, isign=-1, eps=1e-10)
+    ref2 = dense(pts, n_modes, -1) @ fk.reshape(-1)
+    assert rel_l2(c2.cpu().numpy(), ref2.cpu().numpy()) < 1e-9
+    assert rel_l2(c1.cpu().numpy(), c2.cpu().numpy()) > 1e-3
+    c3 = nufft_type2(pts.clone() * 1.05, fk, isign=-1, eps=1e-10)  # a different tensor
+    ref3 = dense(pts * 1.05, n_modes, -1) @ fk.reshape(-1)
+    assert rel_l2(c3.cpu().numpy(), ref3.cpu().numpy()) < 1e-9
