@@ -44,9 +44,9 @@ def candidates(nufft_type, dim):
         bins = [(32, 32, 1), (16, 16, 1), (64, 32, 1), (64, 64, 1), (32, 16, 1)]
     else:
         bins = [(1024, 1, 1), (512, 1, 1), (2048, 1, 1)]
-    # type 2: subproblems of >= 512 / >= 1024 points run 512- / 1024-thread
-    # workgroups
-    subs = (256, 1024, 4096) if nufft_type == 1 else (256, 512, 1024, 2048, 4096)
+    # subproblems of >= 1024 points run 1024-thread workgroups (type 2 also
+    # 512-thread ones from 512 points)
+    subs = (256, 1024, 2048, 4096) if nufft_type == 1 else (256, 512, 1024, 2048, 4096)
     out = []
     for b in bins:
         for msub in subs:

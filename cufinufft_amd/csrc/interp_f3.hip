@@ -1,0 +1,5 @@
+// Interp and coordinate-derivative kernels for float, 3D (one
+// translation unit per precision, dimension and side, so the ns
+// instantiations compile in parallel).
+#include "spreadinterp_impl.inl"
+CFA_INSTANTIATE_INTERP(float, 3)

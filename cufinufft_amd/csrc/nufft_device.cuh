@@ -712,26 +712,34 @@ __device__ __forceinline__ int taps_local_unit(double v, int o, int tw, const Ke
     return l;
 }
 
-// Sum over the 256 threads of the workgroup (returned to all).
-template <typename T>
-__device__ __forceinline__ T block_sum256(T v, T *scratch)
+// Sum over the NW waves of the workgroup (returned to all); scratch holds NW
+// values.
+template <typename T, int NW>
+__device__ __forceinline__ T block_sum(T v, T *scratch)
 {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
     const int wid = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) scratch[wid] = v;
     __syncthreads();
-    const T s = scratch[0] + scratch[1] + scratch[2] + scratch[3];
+    T s = scratch[0];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) s += scratch[w];
     __syncthreads();
     return s;
 }
+template <typename T>
+__device__ __forceinline__ T block_sum256(T v, T *scratch)
+{
+    return block_sum<T, 4>(v, scratch);
+}
 
 // LDS bytes of the fixed-point SM spread: 32- or 64-bit re / im per tile
-// cell, the job's strengths staged as complex T (elem = sizeof(T)), and 4
-// values of reduction scratch.
+// cell, the job's strengths staged as complex T (elem = sizeof(T)), and 16
+// values of reduction scratch (one per wave of a 1024-thread workgroup).
 __host__ __device__ inline size_t fx_spread_lds(size_t cells, int maxsub, int wide, int elem)
 {
-    return cells * (wide ? 16 : 8) + (size_t)maxsub * 2 * elem + 4 * (size_t)elem;
+    return cells * (wide ? 16 : 8) + (size_t)maxsub * 2 * elem + 16 * (size_t)elem;
 }
 
 // Fixed-point accumulator widths: headroom bits of the tile scale, the
@@ -767,7 +775,7 @@ template <> struct FxAcc<long long> {
 // pick 64 bits when it would matter at their tolerance, and FP64 plans always
 // use 64 bits (A * 2^-63 per tap: the rounding level of FP64 sums of size A).
 // The flush multiplies by exp(DIM beta) / S in FP64.
-template <typename T, typename Acc, int DIM, int NS>
+template <typename T, typename Acc, int DIM, int NS, int BT>
 __device__ __forceinline__ void spread_sm_job_fx(const SubprobJob &J, int M, const SortedPts &sp,
                                                  const cpx<T> *__restrict__ c,
                                                  cpx<T> *__restrict__ fw, const BinGeom &g,
@@ -779,21 +787,21 @@ __device__ __forceinline__ void spread_sm_job_fx(const SubprobJob &J, int M, con
     Acc *sim = sre + tl.n;
     cpx<T> *cst = reinterpret_cast<cpx<T> *>(sim + tl.n);  // 2*sizeof(Acc)*tl.n bytes in: aligned
     T *red = reinterpret_cast<T *>(cst + maxsub);
-    for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
+    for (int q = threadIdx.x; q < tl.n; q += BT) {
         sre[q] = 0;
         sim[q] = 0;
     }
     T part = 0;
-    for (int q = threadIdx.x; q < J.np; q += blockDim.x) {
+    for (int q = threadIdx.x; q < J.np; q += BT) {
         const cpx<T> cv = c[(long long)J.t * M + sp.idx[J.p0 + q]];
         cst[q] = cv;
         part += fabs(cv.x) + fabs(cv.y);
     }
-    const T A = block_sum256(part, red);  // also orders the stores above
+    const T A = block_sum<T, BT / 64>(part, red);  // also orders the stores above
     int ex = 0;
     frexp(A, &ex);                        // A < 2^ex
     const int sh = A > T(0) ? FxAcc<Acc>::kBits - ex : 0;  // S = 2^sh, applied with ldexp
-    for (int q = threadIdx.x; q < J.np; q += blockDim.x) {
+    for (int q = threadIdx.x; q < J.np; q += BT) {
         const int i = J.p0 + q;
         T k1[NS], k2[NS], k3[NS];
         const int l1 = taps_local_unit<T, NS>(sp.pos[i], tl.o[0], tl.tw[0], kp, k1);
@@ -825,7 +833,7 @@ __device__ __forceinline__ void spread_sm_job_fx(const SubprobJob &J, int M, con
     // would scale the whole tile by up to 2e-6, and 2^-sh can leave FP32 range
     const double back = ldexp(exp((double)DIM * (double)kp.beta), -sh);
     cpx<T> *f = fw + (long long)J.t * nftot;
-    for (int q = threadIdx.x; q < tl.n; q += blockDim.x) {
+    for (int q = threadIdx.x; q < tl.n; q += BT) {
         const Acc ir = sre[q], ii = sim[q];
         if (ir != 0 || ii != 0) {
             cpx<T> *p = f + tl.grid_offset(q, g);
@@ -836,8 +844,12 @@ __device__ __forceinline__ void spread_sm_job_fx(const SubprobJob &J, int M, con
     __syncthreads();
 }
 
-template <typename T, int DIM, int NS>
-__global__ void __launch_bounds__(256)
+// BT threads per workgroup: 256, or 1024 for large tiles with subproblems
+// of >= 1024 points (a tile over a quarter of the LDS leaves few workgroups
+// per CU, and the wider workgroup keeps the CU's wave count up for the
+// zeroing, the taps and the flush).
+template <typename T, int DIM, int NS, int BT>
+__global__ void __launch_bounds__(BT)
 k_spread_sm(int M, SortedPts sp, const cpx<T> *__restrict__ c, cpx<T> *__restrict__ fw,
             BinGeom g, KerParams<T> kp, SubprobArgs sa, int ntr, long long nftot)
 {
@@ -846,11 +858,11 @@ k_spread_sm(int M, SortedPts sp, const cpx<T> *__restrict__ c, cpx<T> *__restric
     const JobRange jr = xcd_jobs((long long)total * ntr);
     for (long long job = jr.begin; job < jr.end; job += jr.step) {
         if (sizeof(T) == 4 && !kp.fxwide)
-            spread_sm_job_fx<T, int, DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g, kp,
-                                              nftot, smem, sa.maxsub);
+            spread_sm_job_fx<T, int, DIM, NS, BT>(subprob_job(job, total, sa), M, sp, c, fw, g,
+                                                  kp, nftot, smem, sa.maxsub);
         else
-            spread_sm_job_fx<T, long long, DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g,
-                                                    kp, nftot, smem, sa.maxsub);
+            spread_sm_job_fx<T, long long, DIM, NS, BT>(subprob_job(job, total, sa), M, sp, c, fw,
+                                                        g, kp, nftot, smem, sa.maxsub);
     }
 }
 
@@ -910,11 +922,11 @@ k_spread_sm_group(const GroupItem<T> *__restrict__ items, int n,
         const GroupItem<T> &it = items[p];
         const SubprobJob J = subprob_job(job - off[p], it.sa.counters[1], it.sa);
         if (sizeof(T) == 4 && !kp.fxwide)
-            spread_sm_job_fx<T, int, DIM, NS>(J, it.M, it.sp, it.in, it.out, it.g, kp, it.nftot,
-                                              smem, it.sa.maxsub);
+            spread_sm_job_fx<T, int, DIM, NS, 256>(J, it.M, it.sp, it.in, it.out, it.g, kp,
+                                                   it.nftot, smem, it.sa.maxsub);
         else
-            spread_sm_job_fx<T, long long, DIM, NS>(J, it.M, it.sp, it.in, it.out, it.g, kp,
-                                                    it.nftot, smem, it.sa.maxsub);
+            spread_sm_job_fx<T, long long, DIM, NS, 256>(J, it.M, it.sp, it.in, it.out, it.g, kp,
+                                                         it.nftot, smem, it.sa.maxsub);
     }
 }
 

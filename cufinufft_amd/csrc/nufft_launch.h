@@ -29,6 +29,7 @@ struct SortBufs {
     size_t sort_tmp_bytes = 0;
     int *idx = nullptr;         // [M] caller index of each sorted point
     double *pos = nullptr;      // [dim][pos_stride] sorted rescaled positions
+    double *pack = nullptr;     // [M] input-order coordinates, one record per point (2D/3D)
     long long pos_stride = 0;   // capacity in points of each pos plane
     int *counters = nullptr;    // [2]: points, subproblems
     void *scan_part = nullptr;  // int2[ceil(nbins / chunk)]
@@ -52,6 +53,8 @@ int run_radix_sort(int dim, const BinGeom &g, long long nbins, int M, const T *x
                    const Launch &L);
 // Scratch bytes of that sort for M points and nkeys distinct keys.
 size_t radix_tmp_bytes(int M, long long nkeys);
+// Bytes per point of sb.pack for that sort (elem = sizeof(T); 0 in 1D).
+size_t pack_rec_bytes(int dim, int elem);
 // Natural order (gpu_sort = 0): identity index and positions.
 template <typename T>
 int run_natural_order(int dim, const BinGeom &g, int M, const T *x, const T *y, const T *z,

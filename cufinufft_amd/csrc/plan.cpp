@@ -440,6 +440,7 @@ struct Plan {
         sb.sort_tmp = nullptr;
         dfree(sb.idx);
         dfree(sb.pos);
+        dfree(sb.pack);
         dfree(sb.counters);
         if (sb.scan_part) hipFree(sb.scan_part);
         sb.scan_part = nullptr;
@@ -745,8 +746,11 @@ int Plan<T>::setpts(int M_, const T *x, const T *y, const T *z)
         dfree(sb.skey_out);
         dfree(sb.idx);
         dfree(sb.pos);
+        dfree(sb.pack);
+        const size_t pack_doubles = (size_t)M * pack_rec_bytes(dim, sizeof(T)) / 8;
         if (dmalloc(&sb.skey, M) || dmalloc(&sb.sval, M) || dmalloc(&sb.skey_out, M) ||
-            dmalloc(&sb.idx, M) || dmalloc(&sb.pos, (size_t)M * dim)) {
+            dmalloc(&sb.idx, M) || dmalloc(&sb.pos, (size_t)M * dim) ||
+            (pack_doubles && dmalloc(&sb.pack, pack_doubles))) {
             cap_M = -1;
             return kErrHip;
         }

@@ -115,6 +115,65 @@ def test_spread_only(torch_dev, case, method):
     assert scale_err(fw, ref) < (3e-7 if dtype == np.float32 else 1e-12)
 
 
+WIDE_CASES = [
+    # dim, dtype, modes, M, tol, bins: tiles over 40 KiB of LDS
+    (3, np.float32, (24, 20, 28), 20000, 1e-3, (16, 16, 8)),  # 32-bit fixed-point tile
+    (3, np.float32, (24, 20, 28), 20000, 1e-6, (16, 8, 8)),   # 64-bit tile
+    (3, np.float64, (24, 24, 24), 12000, 1e-9, (8, 8, 8)),
+    (2, np.float32, (64, 48), 20000, 1e-4, (64, 64)),
+    (1, np.float64, (100,), 12000, 1e-8, (4096,)),
+]
+
+
+def _spread_lds(dim, ns, bins, elem, maxsub, wide):
+    """LDS bytes of one SM spread workgroup: the padded fixed-point tile, the
+    staged strengths and the reduction scratch."""
+    P = (ns + 1) // 2
+    cells = int(np.prod([b + 2 * P for b in bins[:dim]]))
+    return cells * (16 if wide else 8) + maxsub * 2 * elem + 16 * elem
+
+
+@pytest.mark.parametrize("maxsub", [1024, 2048])
+@pytest.mark.parametrize("case", WIDE_CASES,
+                         ids=lambda c: f"{c[0]}d-{np.dtype(c[1]).name}-{c[4]:g}")
+def test_spread_only_wide_workgroups(torch_dev, case, maxsub):
+    """Large tiles (over a quarter of the 160 KiB LDS) with subproblems of
+    >= 1024 points spread with 1024-thread workgroups; on clustered points the
+    hot bins fill whole subproblems, and at maxsub 2048 each thread takes two
+    points.  Bars: as for the 256-thread spread where the tile is 64-bit; with
+    the 32-bit tile (4 * maxsub * 2^-30 <= tol/5) each tap rounds to 2^-30 of
+    the job's strength mass, so rel-l2 is held to tol/5; that rounding is
+    unbiased noise e, whose least-squares scale error |<e, ref>| / |ref|^2 is
+    about rel-l2 / sqrt(cells it touches), so the scale bar adds four times
+    that to the FP32 kernel-value level."""
+    dim, dtype, nm, M, tol, bins = case
+    _flush(f"spread-only wide {case} maxsub {maxsub}")
+    pts = G.clustered_pts(M, dim, dtype, seed=8, ncl=2, sigma=0.3)
+    c = G.strengths(M, dtype, seed=8)
+    b3 = tuple(bins) + (1,) * (3 - dim)
+    fw, plan = G.run_type1(torch_dev, pts, c, nm, tol, dtype, gpu_method=2,
+                           gpu_spreadinterponly=1, gpu_maxsubprobsize=maxsub,
+                           gpu_binsizex=b3[0], gpu_binsizey=b3[1], gpu_binsizez=b3[2])
+    # some bin holds more than one full subproblem
+    assert plan.sort_arrays()["binsize"].max() > maxsub
+    sp = O.setup_spreader(tol, dtype == np.float32)
+    u32 = dtype == np.float32 and 4 * maxsub * 2.0 ** -30 <= tol / 5
+    elem = np.dtype(dtype).itemsize
+    lds = _spread_lds(dim, sp.ns, bins, elem, maxsub, not u32)
+    assert 160 * 1024 // 4 < lds <= 160 * 1024
+    ref = O.spread(pts, c, list(nm), sp)
+    if dtype == np.float64:
+        bar = 1e-12
+    else:
+        bar = tol / 5 if u32 else 8e-7
+    err = rel_l2(fw, ref)
+    assert err < bar
+    sbar = 3e-7 if dtype == np.float32 else 1e-12
+    if u32:
+        sbar += 4 * err / np.sqrt(np.count_nonzero(ref))
+    assert scale_err(fw, ref) < sbar
+
+
 @pytest.mark.parametrize("method", [1, 2])
 @pytest.mark.parametrize("case", SI_CASES, ids=lambda c: f"{c[0]}d-{np.dtype(c[1]).name}")
 def test_interp_only(torch_dev, case, method):
