@@ -156,11 +156,17 @@ def rel_l2(a, b):
     return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
 
 
+def _layout(opts):
+    return {k: v for k, v in (opts or {}).items() if k != "gpu_kerevalmeth"}
+
+
 def measured_traffic(workload, kernel_prefix, opts):
     """HBM bytes per launch of the kernel for this workload and options, from
     the TRAFFIC records of the committed rocprofv3 summaries in profiles/
     (FETCH_SIZE and WRITE_SIZE runs scaled by a plain-copy calibration), or
-    None when no summary matches."""
+    None when no summary matches. The kernel-evaluation method
+    (exp or polynomial) is left out of the match: it changes arithmetic, not
+    the data the kernel moves."""
     root = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
     if not os.path.isdir(root):
         return None
@@ -172,7 +178,7 @@ def measured_traffic(workload, kernel_prefix, opts):
                 continue
             rec = json.loads(line[len("TRAFFIC "):])
             kern = rec.get("kernel", "")
-            if (rec.get("workload") == workload and rec.get("opts") == opts
+            if (rec.get("workload") == workload and _layout(rec.get("opts")) == _layout(opts)
                     and (kern.startswith(kernel_prefix + ">")
                          or kern.startswith(kernel_prefix + ","))):
                 return float(rec["bytes"])

@@ -845,9 +845,9 @@ __device__ __forceinline__ void spread_sm_job_fx(const SubprobJob &J, int M, con
 }
 
 // BT threads per workgroup: 256, or 1024 for large tiles with subproblems
-// of >= 1024 points (a tile over a quarter of the LDS leaves few workgroups
-// per CU, and the wider workgroup keeps the CU's wave count up for the
-// zeroing, the taps and the flush).
+// of >= 1024 points (a tile over half the LDS leaves one workgroup per CU,
+// and the wider workgroup keeps the CU's wave count up for the zeroing, the
+// taps and the flush).
 template <typename T, int DIM, int NS, int BT>
 __global__ void __launch_bounds__(BT)
 k_spread_sm(int M, SortedPts sp, const cpx<T> *__restrict__ c, cpx<T> *__restrict__ fw,

@@ -75,11 +75,13 @@ int spread_ns(int method, int M, const SortBufs &sb, const cpx<T> *c, cpx<T> *fw
         if (lds > kMaxLdsBytes) return kErrLds;
         int grid = 0;
         SubprobArgs sa{sb.s2b, sb.substart, sb.binstart, sb.binsize, sb.counters, maxsub};
-        // 1024-thread workgroups when a 256-thread one would leave fewer than
-        // 16 waves per CU (tile over a quarter of the LDS) and every thread
-        // has a point (measured: 3D FP32 16x16x8 tiles with 64-bit cells,
-        // 26.5 -> 22.7 ms on 1e8 points; on small 2D tiles 256 threads win)
-        if (maxsub >= 1024 && lds > kMaxLdsBytes / 4) {
+        // 1024-thread workgroups when the tile admits only one workgroup per
+        // CU anyway (over half the LDS), so the wider workgroup adds waves
+        // without taking concurrent jobs away, and subproblems fill it
+        // (measured on 1e8 3D FP32 points: 16x16x8 tiles with 64-bit cells,
+        // 26.5 -> 22.9 ms; 16x16x2 tiles, two per CU at 256 threads but one at
+        // 1024 (the kernel's registers allow 28 waves per CU), 39.7 -> 44.1 ms)
+        if (maxsub >= 1024 && lds > kMaxLdsBytes / 2) {
             int e = sm_grid<k_spread_sm<T, DIM, NS, 1024>>(lds, sub_bound * ntr, L.ncu, &grid,
                                                            1024);
             if (e) return e;
