@@ -35,9 +35,11 @@ struct KerParams {
     // holding the kernel divided by e^beta (peak 1).
     const T *hc;
     int hdeg;
-    // FP32 SM spread: 1 accumulates the LDS tile in 64-bit fixed point, 0 in
-    // 32-bit (chosen per plan from tol and the subproblem size)
+    // FP32 SM spread: 1 sizes the LDS tile for 64-bit fixed point, 0 for
+    // 32-bit (chosen per plan from tol and the subproblem size); jobs of at
+    // most fx32max points accumulate in 32 bits either way, larger ones in 64
     int fxwide;
+    int fx32max;
 };
 
 // Points as the execute kernels read them, in bin-sorted order: idx[i] is
@@ -774,6 +776,9 @@ template <> struct FxAcc<long long> {
 // tap.  With 32 bits that error grows with the points per job (A); FP32 plans
 // pick 64 bits when it would matter at their tolerance, and FP64 plans always
 // use 64 bits (A * 2^-63 per tap: the rounding level of FP64 sums of size A).
+// The width is chosen per job from its point count (A grows with it), so the
+// many small jobs of a plan whose largest ones need 64 bits still take the
+// faster 32-bit atomics.
 // The flush multiplies by exp(DIM beta) / S in FP64.
 template <typename T, typename Acc, int DIM, int NS, int BT>
 __device__ __forceinline__ void spread_sm_job_fx(const SubprobJob &J, int M, const SortedPts &sp,
@@ -857,12 +862,12 @@ k_spread_sm(int M, SortedPts sp, const cpx<T> *__restrict__ c, cpx<T> *__restric
     const int total = sa.counters[1];
     const JobRange jr = xcd_jobs((long long)total * ntr);
     for (long long job = jr.begin; job < jr.end; job += jr.step) {
-        if (sizeof(T) == 4 && !kp.fxwide)
-            spread_sm_job_fx<T, int, DIM, NS, BT>(subprob_job(job, total, sa), M, sp, c, fw, g,
-                                                  kp, nftot, smem, sa.maxsub);
+        const SubprobJob J = subprob_job(job, total, sa);
+        if (sizeof(T) == 4 && J.np <= kp.fx32max)
+            spread_sm_job_fx<T, int, DIM, NS, BT>(J, M, sp, c, fw, g, kp, nftot, smem, sa.maxsub);
         else
-            spread_sm_job_fx<T, long long, DIM, NS, BT>(subprob_job(job, total, sa), M, sp, c, fw,
-                                                        g, kp, nftot, smem, sa.maxsub);
+            spread_sm_job_fx<T, long long, DIM, NS, BT>(J, M, sp, c, fw, g, kp, nftot, smem,
+                                                        sa.maxsub);
     }
 }
 
@@ -921,7 +926,7 @@ k_spread_sm_group(const GroupItem<T> *__restrict__ items, int n,
         const int p = group_member(off, n, job);
         const GroupItem<T> &it = items[p];
         const SubprobJob J = subprob_job(job - off[p], it.sa.counters[1], it.sa);
-        if (sizeof(T) == 4 && !kp.fxwide)
+        if (sizeof(T) == 4 && J.np <= kp.fx32max)
             spread_sm_job_fx<T, int, DIM, NS, 256>(J, it.M, it.sp, it.in, it.out, it.g, kp,
                                                    it.nftot, smem, it.sa.maxsub);
         else

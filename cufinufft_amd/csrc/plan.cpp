@@ -204,11 +204,16 @@ static void fseries(int nf, const Spread<T> &sp, std::vector<double> &out)
 // error grows with the points per subproblem: CPU emulation of the tile on a
 // clustered 3D tol-1e-6 case adds about 2-3 * maxsub * 2^-30 to the relative
 // error of the transform.  The 32-bit tile is kept while 4 * maxsub * 2^-30
-// stays within tol/5; otherwise the tile is 64-bit (ds_add_u64).
-static int fx_wide(int maxsub, double tol)
+// stays within tol/5; otherwise the tile is 64-bit (ds_add_u64).  The rule
+// applies per job: fx32_max is the largest point count that keeps 32 bits,
+// and fx_wide says whether the plan's largest jobs exceed it (which sizes
+// the LDS tile for 64-bit cells).
+static int fx32_max(double tol)
 {
-    return 4.0 * (double)maxsub * std::ldexp(1.0, -30) > tol / 5.0 ? 1 : 0;
+    const double n = std::floor(tol / 5.0 / (4.0 * std::ldexp(1.0, -30)));
+    return n > 2e9 ? 2000000000 : (int)n;
 }
+static int fx_wide(int maxsub, double tol) { return maxsub > fx32_max(tol) ? 1 : 0; }
 
 // ---------------------------------------------------------------------------
 // Piecewise-polynomial ES kernel for gpu_kerevalmeth = 1.  The reference
@@ -364,11 +369,12 @@ struct Plan {
     T *hcoef = nullptr;
     SortBufs sb;
 
-    int fxwide = 0;  // FP32 SM spread tile in 64-bit (1) or 32-bit (0) fixed point
+    int fxwide = 0;   // FP32 SM spread tile sized for 64-bit (1) or 32-bit (0) cells
+    int fx32max = 0;  // FP32: jobs of at most this many points use the 32-bit tile
 
     KerParams<T> ker_params() const
     {
-        return KerParams<T>{sp.beta, sp.c, sp.hw, hcoef, hdeg, fxwide};
+        return KerParams<T>{sp.beta, sp.c, sp.hw, hcoef, hdeg, fxwide, fx32max};
     }
 
     hipfftHandle fft_full = 0, fft_rem = 0;
@@ -604,6 +610,7 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
     // spread tile accumulates in FP64 (FP64 plans) or fixed point (FP32
     // plans); the interp tile holds T.
     fxwide = fx_wide(opts.gpu_maxsubprobsize, (double)tol);
+    fx32max = sizeof(T) == 4 ? fx32_max((double)tol) : 0;
     const int bsv[3] = {opts.gpu_binsizex, opts.gpu_binsizey, opts.gpu_binsizez};
     const size_t tile_bytes =
         type == 1 ? spread_lds_bytes(dim, sp.ns, bsv, sizeof(T), opts.gpu_maxsubprobsize, fxwide)
@@ -1043,7 +1050,10 @@ struct Group {
         int e = 0;
         const Plan<T> &p0 = *plans[0];
         KerParams<T> kp = p0.ker_params();
-        for (const auto *p : plans) kp.fxwide |= p->fxwide;  // the widest member's tile
+        for (const auto *p : plans) {  // the widest member's tile, the strictest 32-bit bound
+            kp.fxwide |= p->fxwide;
+            kp.fx32max = std::min(kp.fx32max, p->fx32max);
+        }
         const int n = (int)plans.size();
         std::vector<hipStream_t> saved(n);
         for (int i = 0; i < n; ++i) {

@@ -174,6 +174,32 @@ def test_spread_only_wide_workgroups(torch_dev, case, maxsub):
     assert scale_err(fw, ref) < sbar
 
 
+def test_spread_only_mixed_width(torch_dev):
+    """One FP32 plan whose jobs take both tile widths: at tol 1e-5 the 32-bit
+    tile holds jobs of up to 536 points (4 * n * 2^-30 <= tol/5), so with
+    1024-point subproblems the sparse bins of the uniform half accumulate in
+    32 bits and the full subproblems of the clusters in 64.  Bars as for the
+    32-bit tile."""
+    dim, dtype, nm, M, tol = 3, np.float32, (40, 36, 32), 40000, 1e-5
+    _flush("spread-only mixed width")
+    u = G.uniform_pts(M // 2, dim, dtype, seed=9)
+    cl = G.clustered_pts(M // 2, dim, dtype, seed=9, ncl=2, sigma=0.2)
+    pts = [np.concatenate([a, b]) for a, b in zip(u, cl)]
+    c = G.strengths(M, dtype, seed=9)
+    fw, plan = G.run_type1(torch_dev, pts, c, nm, tol, dtype, gpu_method=2,
+                           gpu_spreadinterponly=1, gpu_maxsubprobsize=1024, gpu_binsizex=8,
+                           gpu_binsizey=8, gpu_binsizez=8)
+    bsz = plan.sort_arrays()["binsize"]
+    fx32max = int(np.floor(tol / 5 / (4 * 2.0 ** -30)))
+    assert fx32max == 536
+    assert np.any((bsz > 0) & (bsz <= fx32max)) and bsz.max() > 1024
+    sp = O.setup_spreader(tol, True)
+    ref = O.spread(pts, c, list(nm), sp)
+    err = rel_l2(fw, ref)
+    assert err < tol / 5
+    assert scale_err(fw, ref) < 3e-7 + 4 * err / np.sqrt(np.count_nonzero(ref))
+
+
 @pytest.mark.parametrize("method", [1, 2])
 @pytest.mark.parametrize("case", SI_CASES, ids=lambda c: f"{c[0]}d-{np.dtype(c[1]).name}")
 def test_interp_only(torch_dev, case, method):
@@ -256,8 +282,9 @@ def test_type2(torch_dev, case, method, sort):
 @pytest.mark.parametrize("maxsub", [32, 1024])
 def test_type1_fp32_fixed_point_width(torch_dev, maxsub):
     """FP32 SM spread at tol 1e-6 on clustered points: maxsub 32 keeps the
-    32-bit fixed-point tile (4 * 32 * 2^-30 <= tol/5), maxsub 1024 switches to
-    the 64-bit tile (a 32-bit tile would add ~2e-6 here, CPU emulation)."""
+    32-bit fixed-point tile (4 * 32 * 2^-30 <= tol/5); at maxsub 1024 jobs of
+    more than 53 points take the 64-bit tile (a 32-bit tile would add ~2e-6
+    here, CPU emulation), smaller ones stay 32-bit."""
     case = (3, np.float32, (32, 32, 32), 50_000, 1e-6, "clustered")
     dim, dtype, nm, M, tol, dist = case
     _flush(f"fixed-point width {case} maxsub {maxsub}")
