@@ -665,7 +665,63 @@ struct Tile {
         const int gz = DIM > 2 ? wrapmod(o[2] + qz, g.nf[2]) : 0;
         return ((long long)gz * g.nf[1] + gy) * g.nf[0] + gx;
     }
+    // Every tile coordinate o + q of every axis lies in [-nf, 2 nf), so one
+    // compare each way wraps it (false only for tiles wider than the grid).
+    __device__ __forceinline__ bool walkable(const BinGeom &g) const
+    {
+        bool ok = o[0] >= -g.nf[0] && o[0] + tw[0] <= 2 * g.nf[0];
+        if (DIM > 1) ok = ok && o[1] >= -g.nf[1] && o[1] + tw[1] <= 2 * g.nf[1];
+        if (DIM > 2) ok = ok && o[2] >= -g.nf[2] && o[2] + tw[2] <= 2 * g.nf[2];
+        return ok;
+    }
 };
+
+// Calls f(q, fine-grid offset of tile cell q) for q = q0, q0 + step, ... <
+// tl.n.  On walkable tiles the cell's tile coordinates advance by the step's
+// decomposition with carries and wrap with compares, so the loop runs no
+// integer division (six per cell in grid_offset, which dominated the index
+// arithmetic of the tile loads and flushes).
+template <int DIM, int NS, class F>
+__device__ __forceinline__ void for_tile_cells(const Tile<DIM, NS> &tl, const BinGeom &g, int q0,
+                                               int step, F &&f)
+{
+    if (!tl.walkable(g)) {
+        for (int q = q0; q < tl.n; q += step) f(q, tl.grid_offset(q, g));
+        return;
+    }
+    const int t0 = tl.tw[0], t1 = tl.tw[1];
+    int x = q0 % t0, y = (q0 / t0) % t1, z = q0 / (t0 * t1);
+    const int sx = step % t0, sy = (step / t0) % t1, sz = step / (t0 * t1);
+    for (int q = q0; q < tl.n; q += step) {
+        int gx = tl.o[0] + x;
+        gx += gx < 0 ? g.nf[0] : 0;
+        gx -= gx >= g.nf[0] ? g.nf[0] : 0;
+        int gy = 0, gz = 0;
+        if (DIM > 1) {
+            gy = tl.o[1] + y;
+            gy += gy < 0 ? g.nf[1] : 0;
+            gy -= gy >= g.nf[1] ? g.nf[1] : 0;
+        }
+        if (DIM > 2) {
+            gz = tl.o[2] + z;
+            gz += gz < 0 ? g.nf[2] : 0;
+            gz -= gz >= g.nf[2] ? g.nf[2] : 0;
+        }
+        f(q, ((long long)gz * g.nf[1] + gy) * g.nf[0] + gx);
+        // x, y < their widths before the step, so one carry each
+        x += sx;
+        y += sy;
+        z += sz;
+        if (x >= t0) {
+            x -= t0;
+            ++y;
+        }
+        if (y >= t1) {
+            y -= t1;
+            ++z;
+        }
+    }
+}
 
 struct SubprobJob {
     int t, b, p0, np;
@@ -838,14 +894,14 @@ __device__ __forceinline__ void spread_sm_job_fx(const SubprobJob &J, int M, con
     // would scale the whole tile by up to 2e-6, and 2^-sh can leave FP32 range
     const double back = ldexp(exp((double)DIM * (double)kp.beta), -sh);
     cpx<T> *f = fw + (long long)J.t * nftot;
-    for (int q = threadIdx.x; q < tl.n; q += BT) {
+    for_tile_cells(tl, g, threadIdx.x, BT, [&](int q, long long off) {
         const Acc ir = sre[q], ii = sim[q];
         if (ir != 0 || ii != 0) {
-            cpx<T> *p = f + tl.grid_offset(q, g);
+            cpx<T> *p = f + off;
             gadd(&p->x, (T)((double)ir * back));
             gadd(&p->y, (T)((double)ii * back));
         }
-    }
+    });
     __syncthreads();
 }
 
@@ -1248,7 +1304,8 @@ __device__ __forceinline__ void interp_sm_job(const SubprobJob &J, int M, const 
     const Tile<DIM, NS> tl(g, J.b);
     cpx<T> *tile = reinterpret_cast<cpx<T> *>(smem);
     const cpx<T> *f = fw + (long long)J.t * nftot;
-    for (int q = threadIdx.x; q < tl.n; q += blockDim.x) tile[q] = f[tl.grid_offset(q, g)];
+    for_tile_cells(tl, g, threadIdx.x, blockDim.x,
+                   [&](int q, long long off) { tile[q] = f[off]; });
     __syncthreads();
     for (int q = threadIdx.x; q < J.np; q += blockDim.x) {
         const int i = J.p0 + q;
