@@ -74,6 +74,75 @@ __device__ __forceinline__ JobRange xcd_jobs(long long njobs)
     return {b + blockIdx.x / kNumXcd, e, (long long)per};
 }
 
+// Dynamic job queues of the SM kernels.  The jobs are split into kNumXcd
+// contiguous ranges as above, but each range is handed out by an atomic
+// counter, and a workgroup whose own range runs dry takes jobs from the next
+// ranges.  Job costs follow the points per subproblem, so on clustered points
+// equal-count ranges carry very unequal work (1.3-1.6x the mean on the
+// densest range for the 3D workloads); the queues balance it while keeping
+// neighbouring bins on one XCD.  k_jobq_init sets the counters before each
+// launch from the device-side job count.
+static __global__ void k_jobq_init(unsigned long long *__restrict__ jq,
+                                   const int *__restrict__ counters, int ntr)
+{
+    const long long n = (long long)counters[1] * ntr;
+    if (threadIdx.x < kNumXcd)
+        jq[threadIdx.x] = (unsigned long long)(n * threadIdx.x / kNumXcd);
+}
+
+__device__ __forceinline__ long long jobq_end(long long njobs, int x)
+{
+    return njobs * (x + 1) / kNumXcd;
+}
+
+// Next job from queue q onwards (thread 0 only); q and left (queues not yet
+// found empty, counting q) persist across calls, and -1 means all are empty.
+// A counter only grows, so a queue found empty stays empty.
+__device__ __forceinline__ long long jobq_take(unsigned long long *jq, long long njobs, int &q,
+                                               int &left)
+{
+    while (left > 0) {
+        const long long j = (long long)atomicAdd(&jq[q], 1ull);
+        if (j < jobq_end(njobs, q)) return j;
+        q = q + 1 == kNumXcd ? 0 : q + 1;
+        --left;
+    }
+    return -1;
+}
+
+// Runs job(j) for every job handed to this workgroup.  The counter of the
+// next job is bumped before the current one runs and checked after it, so
+// the atomic's latency hides behind the job; job(j) must end with a barrier
+// (every SM job function does).
+template <class F>
+__device__ __forceinline__ void jobq_run(unsigned long long *jq, long long njobs, F &&job)
+{
+    __shared__ long long s_job;
+    int q = blockIdx.x % kNumXcd, left = kNumXcd;  // used by thread 0
+    if (threadIdx.x == 0) s_job = jobq_take(jq, njobs, q, left);
+    __syncthreads();
+    long long j = s_job;
+    while (j >= 0) {
+        unsigned long long raw = 0;
+        if (threadIdx.x == 0 && left > 0) raw = atomicAdd(&jq[q], 1ull);
+        job(j);
+        if (threadIdx.x == 0) {
+            long long nx = -1;
+            if (left > 0) {
+                nx = (long long)raw;
+                if (nx >= jobq_end(njobs, q)) {
+                    q = q + 1 == kNumXcd ? 0 : q + 1;
+                    --left;
+                    nx = jobq_take(jq, njobs, q, left);
+                }
+            }
+            s_job = nx;
+        }
+        __syncthreads();
+        j = s_job;
+    }
+}
+
 // Subproblem bookkeeping written by setpts, read by the SM kernels.
 struct SubprobArgs {
     const int *s2b;       // subproblem -> bin
@@ -912,19 +981,19 @@ __device__ __forceinline__ void spread_sm_job_fx(const SubprobJob &J, int M, con
 template <typename T, int DIM, int NS, int BT>
 __global__ void __launch_bounds__(BT)
 k_spread_sm(int M, SortedPts sp, const cpx<T> *__restrict__ c, cpx<T> *__restrict__ fw,
-            BinGeom g, KerParams<T> kp, SubprobArgs sa, int ntr, long long nftot)
+            BinGeom g, KerParams<T> kp, SubprobArgs sa, int ntr, long long nftot,
+            unsigned long long *jq)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int total = sa.counters[1];
-    const JobRange jr = xcd_jobs((long long)total * ntr);
-    for (long long job = jr.begin; job < jr.end; job += jr.step) {
+    jobq_run(jq, (long long)total * ntr, [&](long long job) {
         const SubprobJob J = subprob_job(job, total, sa);
         if (sizeof(T) == 4 && J.np <= kp.fx32max)
             spread_sm_job_fx<T, int, DIM, NS, BT>(J, M, sp, c, fw, g, kp, nftot, smem, sa.maxsub);
         else
             spread_sm_job_fx<T, long long, DIM, NS, BT>(J, M, sp, c, fw, g, kp, nftot, smem,
                                                         sa.maxsub);
-    }
+    });
 }
 
 // One member of a plan group (many independent plans of one type, dimension,
@@ -1351,13 +1420,14 @@ __device__ __forceinline__ void interp_sm_job(const SubprobJob &J, int M, const 
 template <typename T, int DIM, int NS, int BT>
 __global__ void __launch_bounds__(BT)
 k_interp_sm(int M, SortedPts sp, cpx<T> *__restrict__ c, const cpx<T> *__restrict__ fw,
-            BinGeom g, KerParams<T> kp, SubprobArgs sa, int ntr, long long nftot)
+            BinGeom g, KerParams<T> kp, SubprobArgs sa, int ntr, long long nftot,
+            unsigned long long *jq)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int total = sa.counters[1];
-    const JobRange jr = xcd_jobs((long long)total * ntr);
-    for (long long job = jr.begin; job < jr.end; job += jr.step)
+    jobq_run(jq, (long long)total * ntr, [&](long long job) {
         interp_sm_job<T, DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g, kp, nftot, smem);
+    });
 }
 
 // SM interp of a whole plan group in one persistent launch.

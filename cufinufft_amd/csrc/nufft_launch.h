@@ -14,7 +14,9 @@ template <typename T> struct cpx;
 constexpr int kErrHip = 10;
 constexpr int kErrFft = 11;
 constexpr int kErrLds = 12;
-constexpr size_t kMaxLdsBytes = 160 * 1024;  // LDS of one gfx950 workgroup
+// Dynamic LDS of one gfx950 workgroup: its 160 KiB less room for the
+// kernels' static job slot.
+constexpr size_t kMaxLdsBytes = 160 * 1024 - 64;
 
 // Device arrays written by setpts.
 struct SortBufs {
@@ -32,6 +34,7 @@ struct SortBufs {
     double *pack = nullptr;     // [M] input-order coordinates, one record per point (2D/3D)
     long long pos_stride = 0;   // capacity in points of each pos plane
     int *counters = nullptr;    // [2]: points, subproblems
+    unsigned long long *jobq = nullptr;  // [8] SM job queue counters (one per XCD range)
     void *scan_part = nullptr;  // int2[ceil(nbins / chunk)]
     // fine sort (key = bin * cells-per-bin + cell in bin), when enabled
     int *kcount = nullptr;       // [nkeys]

@@ -448,6 +448,7 @@ struct Plan {
         dfree(sb.pos);
         dfree(sb.pack);
         dfree(sb.counters);
+        dfree(sb.jobq);
         if (sb.scan_part) hipFree(sb.scan_part);
         sb.scan_part = nullptr;
         dfree(sb.kcount);
@@ -676,7 +677,7 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
             return kErrHip;
     }
     if (dmalloc(&sb.binsize, nbins) || dmalloc(&sb.binstart, nbins) ||
-        dmalloc(&sb.substart, nbins + 1) || dmalloc(&sb.counters, 2))
+        dmalloc(&sb.substart, nbins + 1) || dmalloc(&sb.counters, 2) || dmalloc(&sb.jobq, 8))
         return kErrHip;
     const long long nparts = (nbins + kScanChunk - 1) / kScanChunk;
     if (hipMalloc(&sb.scan_part, sizeof(int2) * (size_t)nparts) != hipSuccess) return kErrHip;

@@ -85,13 +85,15 @@ int spread_ns(int method, int M, const SortBufs &sb, const cpx<T> *c, cpx<T> *fw
             int e = sm_grid<k_spread_sm<T, DIM, NS, 1024>>(lds, sub_bound * ntr, L.ncu, &grid,
                                                            1024);
             if (e) return e;
-            k_spread_sm<T, DIM, NS, 1024><<<grid, 1024, lds, L.stream>>>(M, sorted_pts(sb), c, fw,
-                                                                         g, kp, sa, ntr, nftot);
+            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr);
+            k_spread_sm<T, DIM, NS, 1024><<<grid, 1024, lds, L.stream>>>(
+                M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);
         } else {
             int e = sm_grid<k_spread_sm<T, DIM, NS, 256>>(lds, sub_bound * ntr, L.ncu, &grid);
             if (e) return e;
-            k_spread_sm<T, DIM, NS, 256><<<grid, 256, lds, L.stream>>>(M, sorted_pts(sb), c, fw, g,
-                                                                       kp, sa, ntr, nftot);
+            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr);
+            k_spread_sm<T, DIM, NS, 256><<<grid, 256, lds, L.stream>>>(
+                M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);
         }
     }
     return hipGetLastError() == hipSuccess ? 0 : kErrHip;
@@ -114,18 +116,21 @@ int interp_ns(int method, int M, const SortBufs &sb, cpx<T> *c, const cpx<T> *fw
             int e = sm_grid<k_interp_sm<T, DIM, NS, 1024>>(lds, sub_bound * ntr, L.ncu, &grid,
                                                            1024);
             if (e) return e;
-            k_interp_sm<T, DIM, NS, 1024><<<grid, 1024, lds, L.stream>>>(M, sorted_pts(sb), c, fw,
-                                                                         g, kp, sa, ntr, nftot);
+            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr);
+            k_interp_sm<T, DIM, NS, 1024><<<grid, 1024, lds, L.stream>>>(
+                M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);
         } else if (maxsub >= 512) {
             int e = sm_grid<k_interp_sm<T, DIM, NS, 512>>(lds, sub_bound * ntr, L.ncu, &grid, 512);
             if (e) return e;
-            k_interp_sm<T, DIM, NS, 512><<<grid, 512, lds, L.stream>>>(M, sorted_pts(sb), c, fw, g,
-                                                                       kp, sa, ntr, nftot);
+            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr);
+            k_interp_sm<T, DIM, NS, 512><<<grid, 512, lds, L.stream>>>(
+                M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);
         } else {
             int e = sm_grid<k_interp_sm<T, DIM, NS, 256>>(lds, sub_bound * ntr, L.ncu, &grid);
             if (e) return e;
-            k_interp_sm<T, DIM, NS, 256><<<grid, 256, lds, L.stream>>>(M, sorted_pts(sb), c, fw, g,
-                                                                       kp, sa, ntr, nftot);
+            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr);
+            k_interp_sm<T, DIM, NS, 256><<<grid, 256, lds, L.stream>>>(
+                M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);
         }
     }
     return hipGetLastError() == hipSuccess ? 0 : kErrHip;
