@@ -134,7 +134,7 @@ def tune(nufft_type, modes, pts, eps, dtype=np.float32, n_trans=1, maxbatchsize=
     table = []
     best, best_t = None, float("inf")
 
-    def trial(opts):
+    def trial(opts, n=reps):
         kw = dict(upsampfac=upsampfac)
         kw.update(opts)
         plan = None
@@ -145,10 +145,10 @@ def tune(nufft_type, modes, pts, eps, dtype=np.float32, n_trans=1, maxbatchsize=
             plan.execute(*args)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            for _ in range(reps):
+            for _ in range(n):
                 plan.execute(*args)
             torch.cuda.synchronize()
-            t = (time.perf_counter() - t0) / reps * 1e3
+            t = (time.perf_counter() - t0) / n * 1e3
         except RuntimeError as e:
             if log:
                 log(f"  skip {opts}: {e}")
@@ -167,14 +167,23 @@ def tune(nufft_type, modes, pts, eps, dtype=np.float32, n_trans=1, maxbatchsize=
     if best is None:
         raise RuntimeError("no candidate option set could be planned")
     if cands is None:
-        # second stage on the winner: the polynomial kernel (sigma = 2 only)
-        # and the smaller sigma = 1.25 grid (wider kernel, smaller FFT)
+        # The short first-stage timings are noisy at the few-percent level:
+        # re-time the three fastest with more executes and, on the new
+        # winner, the polynomial kernel (sigma = 2 only) and the smaller
+        # sigma = 1.25 grid (wider kernel, smaller FFT).
+        long_reps = max(3 * reps, 5)
+        top = sorted(table, key=lambda r: r[1])[:3]
+        best, best_t = None, float("inf")
+        for opts, _ in top:
+            t = trial(opts, long_reps)
+            if t is not None and t < best_t:
+                best, best_t = opts, t
         extra = []
         if upsampfac == 2.0:
             extra.append(dict(best, gpu_kerevalmeth=1))
             extra.append(dict(best, upsampfac=1.25))
         for opts in extra:
-            t = trial(opts)
+            t = trial(opts, long_reps)
             if t is not None and t < best_t:
                 best, best_t = opts, t
     _CACHE[key] = best

@@ -74,20 +74,33 @@ __device__ __forceinline__ JobRange xcd_jobs(long long njobs)
     return {b + blockIdx.x / kNumXcd, e, (long long)per};
 }
 
-// Dynamic job queues of the SM kernels.  The jobs are split into kNumXcd
-// contiguous ranges as above, but each range is handed out by an atomic
-// counter, and a workgroup whose own range runs dry takes jobs from the next
+// Job scheduling of the SM kernels.  The jobs are split into kNumXcd
+// contiguous ranges as above.  A workgroup first runs its static share of
+// the first half of its XCD's range (every (grid/kNumXcd)-th job, no
+// atomics), then takes jobs one by one from an atomic counter over the rest
+// of the range, and when that runs dry, from the counters of the next
 // ranges.  Job costs follow the points per subproblem, so on clustered points
 // equal-count ranges carry very unequal work (1.3-1.6x the mean on the
-// densest range for the 3D workloads); the queues balance it while keeping
-// neighbouring bins on one XCD.  k_jobq_init sets the counters before each
-// launch from the device-side job count.
+// densest range for the 3D workloads): the dynamic halves balance that while
+// neighbouring bins stay on one XCD.  The static half keeps the counters out
+// of the way of balanced workloads (on uniform points a counter per job cost
+// ~10%: every workgroup of an XCD hits one address at the same moments).
+// k_jobq_init sets the counters before each launch from the device-side job
+// count and the launch's workgroups per XCD (0 when the grid is not a
+// multiple of kNumXcd: then every job is dynamic).
+__device__ __forceinline__ long long jobq_static(long long njobs, int x, int per)
+{
+    if (per <= 0) return 0;
+    const long long n = njobs * (x + 1) / kNumXcd - njobs * x / kNumXcd;
+    return n / 2 / per * per;
+}
+
 static __global__ void k_jobq_init(unsigned long long *__restrict__ jq,
-                                   const int *__restrict__ counters, int ntr)
+                                   const int *__restrict__ counters, int ntr, int per)
 {
     const long long n = (long long)counters[1] * ntr;
-    if (threadIdx.x < kNumXcd)
-        jq[threadIdx.x] = (unsigned long long)(n * threadIdx.x / kNumXcd);
+    const int x = threadIdx.x;
+    if (x < kNumXcd) jq[x] = (unsigned long long)(n * x / kNumXcd + jobq_static(n, x, per));
 }
 
 __device__ __forceinline__ long long jobq_end(long long njobs, int x)
@@ -110,15 +123,19 @@ __device__ __forceinline__ long long jobq_take(unsigned long long *jq, long long
     return -1;
 }
 
-// Runs job(j) for every job handed to this workgroup.  The counter of the
-// next job is bumped before the current one runs and checked after it, so
-// the atomic's latency hides behind the job; job(j) must end with a barrier
-// (every SM job function does).
+// Runs job(j) for every job handed to this workgroup: the static share, then
+// the dynamic ones.  The counter of the next dynamic job is bumped before the
+// current one runs and checked after it, so the atomic's latency overlaps
+// the job; job(j) must end with a barrier (every SM job function does).
 template <class F>
 __device__ __forceinline__ void jobq_run(unsigned long long *jq, long long njobs, F &&job)
 {
     __shared__ long long s_job;
-    int q = blockIdx.x % kNumXcd, left = kNumXcd;  // used by thread 0
+    const int x = blockIdx.x % kNumXcd;
+    const int per = gridDim.x % kNumXcd == 0 ? (int)(gridDim.x / kNumXcd) : 0;
+    const long long b = njobs * x / kNumXcd, ns = jobq_static(njobs, x, per);
+    for (long long j = b + blockIdx.x / kNumXcd; j < b + ns; j += per) job(j);
+    int q = x, left = kNumXcd;  // used by thread 0
     if (threadIdx.x == 0) s_job = jobq_take(jq, njobs, q, left);
     __syncthreads();
     long long j = s_job;

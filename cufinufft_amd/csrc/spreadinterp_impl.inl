@@ -51,6 +51,10 @@ int sm_grid(size_t lds, long long jobs, int ncu, int *grid, int threads = 256)
     return 0;
 }
 
+// Workgroups per XCD of a persistent launch for the static job share (0:
+// grid not a multiple of the XCD count, every job dynamic).
+inline int jobq_per(int grid) { return grid % kNumXcd == 0 ? grid / kNumXcd : 0; }
+
 inline SortedPts sorted_pts(const SortBufs &sb) { return SortedPts{sb.idx, sb.pos, sb.pos_stride}; }
 
 template <typename T, int DIM, int NS>
@@ -85,13 +89,13 @@ int spread_ns(int method, int M, const SortBufs &sb, const cpx<T> *c, cpx<T> *fw
             int e = sm_grid<k_spread_sm<T, DIM, NS, 1024>>(lds, sub_bound * ntr, L.ncu, &grid,
                                                            1024);
             if (e) return e;
-            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr);
+            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr, jobq_per(grid));
             k_spread_sm<T, DIM, NS, 1024><<<grid, 1024, lds, L.stream>>>(
                 M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);
         } else {
             int e = sm_grid<k_spread_sm<T, DIM, NS, 256>>(lds, sub_bound * ntr, L.ncu, &grid);
             if (e) return e;
-            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr);
+            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr, jobq_per(grid));
             k_spread_sm<T, DIM, NS, 256><<<grid, 256, lds, L.stream>>>(
                 M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);
         }
@@ -116,19 +120,19 @@ int interp_ns(int method, int M, const SortBufs &sb, cpx<T> *c, const cpx<T> *fw
             int e = sm_grid<k_interp_sm<T, DIM, NS, 1024>>(lds, sub_bound * ntr, L.ncu, &grid,
                                                            1024);
             if (e) return e;
-            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr);
+            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr, jobq_per(grid));
             k_interp_sm<T, DIM, NS, 1024><<<grid, 1024, lds, L.stream>>>(
                 M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);
         } else if (maxsub >= 512) {
             int e = sm_grid<k_interp_sm<T, DIM, NS, 512>>(lds, sub_bound * ntr, L.ncu, &grid, 512);
             if (e) return e;
-            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr);
+            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr, jobq_per(grid));
             k_interp_sm<T, DIM, NS, 512><<<grid, 512, lds, L.stream>>>(
                 M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);
         } else {
             int e = sm_grid<k_interp_sm<T, DIM, NS, 256>>(lds, sub_bound * ntr, L.ncu, &grid);
             if (e) return e;
-            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr);
+            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr, jobq_per(grid));
             k_interp_sm<T, DIM, NS, 256><<<grid, 256, lds, L.stream>>>(
                 M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);
         }
