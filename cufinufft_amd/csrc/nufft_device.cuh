@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "nufft_launch.h"
+
 namespace cfa {
 
 template <typename T>
@@ -85,78 +87,98 @@ __device__ __forceinline__ JobRange xcd_jobs(long long njobs)
 // neighbouring bins stay on one XCD.  The static half keeps the counters out
 // of the way of balanced workloads (on uniform points a counter per job cost
 // ~10%: every workgroup of an XCD hits one address at the same moments).
-// k_jobq_init sets the counters before each launch from the device-side job
-// count and the launch's workgroups per XCD (0 when the grid is not a
-// multiple of kNumXcd: then every job is dynamic).
+// Launches with fewer than 4 jobs per workgroup, or a grid that is not a
+// multiple of kNumXcd, keep the static split alone (the probes of a dry
+// queue cost more than any balance gained there).
+//
+// jq holds kNumXcd counters of dynamic jobs taken (0-based) and a count of
+// finished workgroups; they are zero between launches: the last workgroup
+// to finish clears them, so a launch needs no set-up kernel (and a captured
+// graph replays correctly).  Launches sharing one jq must be serialised (one
+// plan's stream, or one group's).
+static_assert(kJobqWords == kNumXcd + 1, "job queue: a counter per XCD range + a done count");
+
 __device__ __forceinline__ long long jobq_static(long long njobs, int x, int per)
 {
-    if (per <= 0) return 0;
     const long long n = njobs * (x + 1) / kNumXcd - njobs * x / kNumXcd;
     return n / 2 / per * per;
 }
 
-static __global__ void k_jobq_init(unsigned long long *__restrict__ jq,
-                                   const int *__restrict__ counters, int ntr, int per)
-{
-    const long long n = (long long)counters[1] * ntr;
-    const int x = threadIdx.x;
-    if (x < kNumXcd) jq[x] = (unsigned long long)(n * x / kNumXcd + jobq_static(n, x, per));
-}
-
-__device__ __forceinline__ long long jobq_end(long long njobs, int x)
-{
-    return njobs * (x + 1) / kNumXcd;
-}
-
-// Next job from queue q onwards (thread 0 only); q and left (queues not yet
-// found empty, counting q) persist across calls, and -1 means all are empty.
-// A counter only grows, so a queue found empty stays empty.
-__device__ __forceinline__ long long jobq_take(unsigned long long *jq, long long njobs, int &q,
-                                               int &left)
+// Next dynamic job from queue q onwards (thread 0 only); q and left (queues
+// not yet found empty, counting q) persist across calls, and -1 means all are
+// empty.  A counter only grows, so a queue found empty stays empty.
+__device__ __forceinline__ long long jobq_take(unsigned long long *jq, long long njobs, int per,
+                                               int &q, int &left)
 {
     while (left > 0) {
-        const long long j = (long long)atomicAdd(&jq[q], 1ull);
-        if (j < jobq_end(njobs, q)) return j;
+        const long long t = (long long)atomicAdd(&jq[q], 1ull);
+        const long long j = njobs * q / kNumXcd + jobq_static(njobs, q, per) + t;
+        if (j < njobs * (q + 1) / kNumXcd) return j;
         q = q + 1 == kNumXcd ? 0 : q + 1;
         --left;
     }
     return -1;
 }
 
-// Runs job(j) for every job handed to this workgroup: the static share, then
-// the dynamic ones.  The counter of the next dynamic job is bumped before the
-// current one runs and checked after it, so the atomic's latency overlaps
-// the job; job(j) must end with a barrier (every SM job function does).
+// Runs job(j) for every job handed to this workgroup: the static sequence,
+// then the dynamic jobs.  The counter of the next dynamic job is bumped
+// before the current one runs and checked after it, so the atomic's latency
+// overlaps the job; job(j) must end with a barrier (every SM job function
+// does).  job is called from one place only, so the job body is inlined
+// once.
 template <class F>
 __device__ __forceinline__ void jobq_run(unsigned long long *jq, long long njobs, F &&job)
 {
     __shared__ long long s_job;
-    const int x = blockIdx.x % kNumXcd;
-    const int per = gridDim.x % kNumXcd == 0 ? (int)(gridDim.x / kNumXcd) : 0;
-    const long long b = njobs * x / kNumXcd, ns = jobq_static(njobs, x, per);
-    for (long long j = b + blockIdx.x / kNumXcd; j < b + ns; j += per) job(j);
-    int q = x, left = kNumXcd;  // used by thread 0
-    if (threadIdx.x == 0) s_job = jobq_take(jq, njobs, q, left);
-    __syncthreads();
-    long long j = s_job;
+    const int G = gridDim.x;
+    const bool dyn = G % kNumXcd == 0 && njobs >= 4ll * G;
+    const int x = blockIdx.x % kNumXcd, per = G / kNumXcd;
+    long long j, se, step;  // static sequence j, j + step, ... < se
+    if (dyn) {
+        const long long b = njobs * x / kNumXcd;
+        j = b + blockIdx.x / kNumXcd;
+        se = b + jobq_static(njobs, x, per);
+        step = per;
+    } else {
+        const JobRange jr = xcd_jobs(njobs);
+        j = jr.begin;
+        se = jr.end;
+        step = jr.step;
+    }
+    int q = x, left = dyn ? kNumXcd : 0;  // dynamic state, used by thread 0
+    bool stat = j < se;
+    if (!stat) {
+        if (threadIdx.x == 0) s_job = jobq_take(jq, njobs, per, q, left);
+        __syncthreads();
+        j = s_job;
+    }
     while (j >= 0) {
         unsigned long long raw = 0;
-        if (threadIdx.x == 0 && left > 0) raw = atomicAdd(&jq[q], 1ull);
+        if (!stat && threadIdx.x == 0) raw = atomicAdd(&jq[q], 1ull);
         job(j);
-        if (threadIdx.x == 0) {
-            long long nx = -1;
-            if (left > 0) {
-                nx = (long long)raw;
-                if (nx >= jobq_end(njobs, q)) {
-                    q = q + 1 == kNumXcd ? 0 : q + 1;
-                    --left;
-                    nx = jobq_take(jq, njobs, q, left);
-                }
+        if (stat) {
+            j += step;
+            if (j < se) continue;
+            stat = false;
+            if (threadIdx.x == 0) s_job = jobq_take(jq, njobs, per, q, left);
+        } else if (threadIdx.x == 0) {
+            long long nx = njobs * q / kNumXcd + jobq_static(njobs, q, per) + (long long)raw;
+            if (nx >= njobs * (q + 1) / kNumXcd) {
+                q = q + 1 == kNumXcd ? 0 : q + 1;
+                --left;
+                nx = jobq_take(jq, njobs, per, q, left);
             }
             s_job = nx;
         }
         __syncthreads();
         j = s_job;
+    }
+    // every queue is dry for this workgroup; the last one to get here clears
+    // the words for the next launch (all others are done with them)
+    if (dyn && threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&jq[kNumXcd], 1ull) == (unsigned long long)G - 1)
+            for (int k = 0; k < kJobqWords; ++k) atomicExch(&jq[k], 0ull);
     }
 }
 
@@ -1060,11 +1082,10 @@ __device__ __forceinline__ int group_member(const long long *off, int n, long lo
 template <typename T, int DIM, int NS>
 __global__ void __launch_bounds__(256)
 k_spread_sm_group(const GroupItem<T> *__restrict__ items, int n,
-                  const long long *__restrict__ off, KerParams<T> kp)
+                  const long long *__restrict__ off, KerParams<T> kp, unsigned long long *jq)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const JobRange jr = xcd_jobs(off[n]);
-    for (long long job = jr.begin; job < jr.end; job += jr.step) {
+    jobq_run(jq, off[n], [&](long long job) {
         const int p = group_member(off, n, job);
         const GroupItem<T> &it = items[p];
         const SubprobJob J = subprob_job(job - off[p], it.sa.counters[1], it.sa);
@@ -1074,7 +1095,7 @@ k_spread_sm_group(const GroupItem<T> *__restrict__ items, int n,
         else
             spread_sm_job_fx<T, long long, DIM, NS, 256>(J, it.M, it.sp, it.in, it.out, it.g, kp,
                                                          it.nftot, smem, it.sa.maxsub);
-    }
+    });
 }
 
 // ===========================================================================
@@ -1451,16 +1472,15 @@ k_interp_sm(int M, SortedPts sp, cpx<T> *__restrict__ c, const cpx<T> *__restric
 template <typename T, int DIM, int NS>
 __global__ void __launch_bounds__(256)
 k_interp_sm_group(const GroupItem<T> *__restrict__ items, int n,
-                  const long long *__restrict__ off, KerParams<T> kp)
+                  const long long *__restrict__ off, KerParams<T> kp, unsigned long long *jq)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const JobRange jr = xcd_jobs(off[n]);
-    for (long long job = jr.begin; job < jr.end; job += jr.step) {
+    jobq_run(jq, off[n], [&](long long job) {
         const int p = group_member(off, n, job);
         const GroupItem<T> &it = items[p];
         const SubprobJob J = subprob_job(job - off[p], it.sa.counters[1], it.sa);
         interp_sm_job<T, DIM, NS>(J, it.M, it.sp, it.out, it.in, it.g, kp, it.nftot, smem);
-    }
+    });
 }
 // ===========================================================================
 // execute: deconvolve (type 1) / amplify (type 2), one fused pass each

@@ -12,6 +12,7 @@ template <typename T> struct KerParams;
 template <typename T> struct cpx;
 
 constexpr int kErrHip = 10;
+constexpr int kJobqWords = 9;  // SM job queue: one counter per XCD range + finished workgroups
 constexpr int kErrFft = 11;
 constexpr int kErrLds = 12;
 // Dynamic LDS of one gfx950 workgroup: its 160 KiB less room for the
@@ -34,7 +35,7 @@ struct SortBufs {
     double *pack = nullptr;     // [M] input-order coordinates, one record per point (2D/3D)
     long long pos_stride = 0;   // capacity in points of each pos plane
     int *counters = nullptr;    // [2]: points, subproblems
-    unsigned long long *jobq = nullptr;  // [8] SM job queue counters (one per XCD range)
+    unsigned long long *jobq = nullptr;  // [kJobqWords] SM job queue words, zero between launches
     void *scan_part = nullptr;  // int2[ceil(nbins / chunk)]
     // fine sort (key = bin * cells-per-bin + cell in bin), when enabled
     int *kcount = nullptr;       // [nkeys]

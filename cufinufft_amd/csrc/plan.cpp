@@ -677,7 +677,9 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
             return kErrHip;
     }
     if (dmalloc(&sb.binsize, nbins) || dmalloc(&sb.binstart, nbins) ||
-        dmalloc(&sb.substart, nbins + 1) || dmalloc(&sb.counters, 2) || dmalloc(&sb.jobq, 8))
+        dmalloc(&sb.substart, nbins + 1) || dmalloc(&sb.counters, 2) ||
+        dmalloc(&sb.jobq, kJobqWords) ||
+        hipMemset(sb.jobq, 0, kJobqWords * sizeof(unsigned long long)) != hipSuccess)
         return kErrHip;
     const long long nparts = (nbins + kScanChunk - 1) / kScanChunk;
     if (hipMalloc(&sb.scan_part, sizeof(int2) * (size_t)nparts) != hipSuccess) return kErrHip;
@@ -953,7 +955,7 @@ struct Group {
     int maxsub = 1;
     long long jobs_bound = 0;
     void *items = nullptr;
-    long long *off = nullptr;
+    long long *off = nullptr;  // n + 1 job offsets, then the job-queue words (zeroed)
     hipStream_t stream = nullptr, user = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     hipGraph_t graph = nullptr;
@@ -1038,7 +1040,8 @@ struct Group {
                                       p.M, p.ntransf);
         }
         if (hipMalloc(&items, h.size()) != hipSuccess ||
-            hipMalloc((void **)&off, sizeof(long long) * (n + 1)) != hipSuccess)
+            hipMalloc((void **)&off, sizeof(long long) * (n + 1 + kJobqWords)) != hipSuccess ||
+            hipMemset(off + n + 1, 0, sizeof(long long) * kJobqWords) != hipSuccess)
             return kErrHip;
         if (hipMemcpy(items, h.data(), h.size(), hipMemcpyHostToDevice) != hipSuccess)
             return kErrHip;

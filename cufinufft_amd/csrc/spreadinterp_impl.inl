@@ -51,10 +51,6 @@ int sm_grid(size_t lds, long long jobs, int ncu, int *grid, int threads = 256)
     return 0;
 }
 
-// Workgroups per XCD of a persistent launch for the static job share (0:
-// grid not a multiple of the XCD count, every job dynamic).
-inline int jobq_per(int grid) { return grid % kNumXcd == 0 ? grid / kNumXcd : 0; }
-
 inline SortedPts sorted_pts(const SortBufs &sb) { return SortedPts{sb.idx, sb.pos, sb.pos_stride}; }
 
 template <typename T, int DIM, int NS>
@@ -89,13 +85,11 @@ int spread_ns(int method, int M, const SortBufs &sb, const cpx<T> *c, cpx<T> *fw
             int e = sm_grid<k_spread_sm<T, DIM, NS, 1024>>(lds, sub_bound * ntr, L.ncu, &grid,
                                                            1024);
             if (e) return e;
-            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr, jobq_per(grid));
             k_spread_sm<T, DIM, NS, 1024><<<grid, 1024, lds, L.stream>>>(
                 M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);
         } else {
             int e = sm_grid<k_spread_sm<T, DIM, NS, 256>>(lds, sub_bound * ntr, L.ncu, &grid);
             if (e) return e;
-            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr, jobq_per(grid));
             k_spread_sm<T, DIM, NS, 256><<<grid, 256, lds, L.stream>>>(
                 M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);
         }
@@ -120,19 +114,16 @@ int interp_ns(int method, int M, const SortBufs &sb, cpx<T> *c, const cpx<T> *fw
             int e = sm_grid<k_interp_sm<T, DIM, NS, 1024>>(lds, sub_bound * ntr, L.ncu, &grid,
                                                            1024);
             if (e) return e;
-            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr, jobq_per(grid));
             k_interp_sm<T, DIM, NS, 1024><<<grid, 1024, lds, L.stream>>>(
                 M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);
         } else if (maxsub >= 512) {
             int e = sm_grid<k_interp_sm<T, DIM, NS, 512>>(lds, sub_bound * ntr, L.ncu, &grid, 512);
             if (e) return e;
-            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr, jobq_per(grid));
             k_interp_sm<T, DIM, NS, 512><<<grid, 512, lds, L.stream>>>(
                 M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);
         } else {
             int e = sm_grid<k_interp_sm<T, DIM, NS, 256>>(lds, sub_bound * ntr, L.ncu, &grid);
             if (e) return e;
-            k_jobq_init<<<1, 64, 0, L.stream>>>(sb.jobq, sb.counters, ntr, jobq_per(grid));
             k_interp_sm<T, DIM, NS, 256><<<grid, 256, lds, L.stream>>>(
                 M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);
         }
@@ -145,20 +136,23 @@ int group_ns(int type, const GroupItem<T> *items, int n, const long long *off,
              const KerParams<T> &kp, const int *bs, int maxsub, long long jobs_bound,
              const Launch &L)
 {
+    // the job queue words follow the n + 1 offsets in the same buffer
+    unsigned long long *jq =
+        reinterpret_cast<unsigned long long *>(const_cast<long long *>(off) + n + 1);
     if (type == 1) {
         const size_t lds = spread_lds_bytes(DIM, NS, bs, sizeof(T), maxsub, kp.fxwide);
         if (lds > kMaxLdsBytes) return kErrLds;
         int grid = 0;
         int e = sm_grid<k_spread_sm_group<T, DIM, NS>>(lds, jobs_bound, L.ncu, &grid);
         if (e) return e;
-        k_spread_sm_group<T, DIM, NS><<<grid, 256, lds, L.stream>>>(items, n, off, kp);
+        k_spread_sm_group<T, DIM, NS><<<grid, 256, lds, L.stream>>>(items, n, off, kp, jq);
     } else {
         const size_t lds = sm_tile_bytes(DIM, NS, bs, sizeof(T));
         if (lds > kMaxLdsBytes) return kErrLds;
         int grid = 0;
         int e = sm_grid<k_interp_sm_group<T, DIM, NS>>(lds, jobs_bound, L.ncu, &grid);
         if (e) return e;
-        k_interp_sm_group<T, DIM, NS><<<grid, 256, lds, L.stream>>>(items, n, off, kp);
+        k_interp_sm_group<T, DIM, NS><<<grid, 256, lds, L.stream>>>(items, n, off, kp, jq);
     }
     return hipGetLastError() == hipSuccess ? 0 : kErrHip;
 }

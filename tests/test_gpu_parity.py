@@ -279,6 +279,54 @@ def test_type2(torch_dev, case, method, sort):
     assert rel_l2(c, oc) <= tol / 5
 
 
+@pytest.mark.parametrize("nufft_type", [1, 2])
+def test_dynamic_job_queues_repeated(torch_dev, nufft_type):
+    """Launches with at least 4 jobs per workgroup take the second half of
+    each XCD's job range from atomic counters, which the last workgroup
+    clears for the next launch.  Large tiles (one workgroup per CU, so a
+    256-workgroup grid) with 64-point subproblems give ~1800 jobs; three
+    executes in a row must all match the oracle (a counter left dirty would
+    skip jobs on the next launch) and each other."""
+    import torch
+    from cufinufft_amd import cufinufft
+    dim, nm, M = 3, (32, 32, 32), 100_000
+    if nufft_type == 1:
+        dtype, tol, bins = np.float32, 1e-6, (16, 16, 8)    # 64-bit tile, 147 KiB
+    else:
+        dtype, tol, bins = np.float64, 1e-9, (16, 16, 4)    # FP64 tile, 151 KiB
+    _flush(f"dynamic job queues type {nufft_type}")
+    pts = G.clustered_pts(M, dim, dtype, seed=51)
+    opts = dict(gpu_method=2, gpu_maxsubprobsize=64, gpu_binsizex=bins[0],
+                gpu_binsizey=bins[1], gpu_binsizez=bins[2])
+    plan = cufinufft(nufft_type, nm[::-1], eps=tol, dtype=dtype, **opts)
+    plan.set_pts(*[G.to_dev(p, torch_dev) for p in pts[::-1]])
+    assert plan.info()["nsubprob"] >= 4 * 256
+    P = O.Plan(nufft_type, list(nm), tol, single=dtype == np.float32)
+    outs = []
+    if nufft_type == 1:
+        c = G.strengths(M, dtype, seed=51)
+        dc = G.to_dev(c, torch_dev)
+        ref = P.type1(pts, c)
+        for _ in range(3):
+            fk = torch.zeros(nm[::-1], dtype=dc.dtype, device=torch_dev)
+            plan.execute(dc, fk)
+            outs.append(fk.cpu().numpy())
+    else:
+        fk = G.modes(nm[::-1], dtype, seed=51)
+        dfk = G.to_dev(fk, torch_dev)
+        ref = P.type2(pts, fk)
+        for _ in range(3):
+            cc = torch.zeros(M, dtype=dfk.dtype, device=torch_dev)
+            plan.execute(cc, dfk)
+            outs.append(cc.cpu().numpy())
+    # FP32: the FP32 kernel values alone cost ~5.5e-7 in 3D (as for the
+    # spread-only bar); a skipped or repeated job costs orders of magnitude more
+    bar = 8e-7 if dtype == np.float32 else tol / 5
+    for o in outs:
+        assert rel_l2(o, ref) <= bar
+        assert rel_l2(o, outs[0]) <= (1e-6 if dtype == np.float32 else 1e-13)
+
+
 @pytest.mark.parametrize("maxsub", [32, 1024])
 def test_type1_fp32_fixed_point_width(torch_dev, maxsub):
     """FP32 SM spread at tol 1e-6 on clustered points: maxsub 32 keeps the
