@@ -7,6 +7,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstddef>
+#include <type_traits>
+
 #include "nufft_launch.h"
 
 namespace cfa {
@@ -126,12 +129,42 @@ __device__ __forceinline__ long long jobq_take(unsigned long long *jq, long long
 // overlaps the job; job(j) must end with a barrier (every SM job function
 // does).  job is called from one place only, so the job body is inlined
 // once.
-template <class F>
-__device__ __forceinline__ void jobq_run(unsigned long long *jq, long long njobs, F &&job)
+//
+// pos(j) (optional) gives the sorted-point position of job j's first point
+// over all transforms (monotone in j; npts = the position past the last
+// job).  With it the workgroups first estimate the work of each XCD range
+// (points plus a per-job overhead of 8 points, from 9 lookups spread over
+// the lanes of wave 0) and keep the static split when no range exceeds the
+// mean by more than 10%: uniform points pay no counters at all.
+template <class F, class Pos = std::nullptr_t>
+__device__ __forceinline__ void jobq_run(unsigned long long *jq, long long njobs, F &&job,
+                                         Pos &&pos = nullptr, long long npts = 0)
 {
     __shared__ long long s_job;
     const int G = gridDim.x;
-    const bool dyn = G % kNumXcd == 0 && njobs >= 4ll * G;
+    bool dyn = G % kNumXcd == 0 && njobs >= 4ll * G;
+    if constexpr (!std::is_same_v<std::decay_t<Pos>, std::nullptr_t>) {
+        if (dyn) {
+            if (threadIdx.x < 64) {
+                const int k = threadIdx.x;
+                long long w = 0;
+                if (k < kNumXcd) {
+                    const long long b = njobs * k / kNumXcd, e = njobs * (k + 1) / kNumXcd;
+                    if (e > b) w = (e < njobs ? pos(e) : npts) - pos(b) + 8 * (e - b);
+                }
+                long long mx = w, sum = w;
+#pragma unroll
+                for (int d = 4; d >= 1; d >>= 1) {
+                    mx = max(mx, (long long)__shfl_xor(mx, d, 64));
+                    sum += (long long)__shfl_xor(sum, d, 64);
+                }
+                if (threadIdx.x == 0) s_job = 80 * mx > 11 * sum ? 1 : 0;
+            }
+            __syncthreads();
+            dyn = s_job != 0;
+            __syncthreads();
+        }
+    }
     const int x = blockIdx.x % kNumXcd, per = G / kNumXcd;
     long long j, se, step;  // static sequence j, j + step, ... < se
     if (dyn) {
@@ -847,6 +880,17 @@ __device__ __forceinline__ SubprobJob subprob_job(long long job, int total, cons
     return J;
 }
 
+// Position of job's first point in the sorted points of all transforms
+// (transform-major), for the work estimate of the job scheduler.
+__device__ __forceinline__ long long subprob_pos(long long job, int total, int M,
+                                                 const SubprobArgs &sa)
+{
+    const int t = (int)(job / total);
+    const int s = (int)(job - (long long)t * total);
+    const int b = sa.s2b[s];
+    return (long long)t * M + sa.binstart[b] + (long long)(s - sa.substart[b]) * sa.maxsub;
+}
+
 // SM spread (Spread_{2,3}d_Subprob, spreadinterp2d.cu:152-231,
 // spreadinterp3d.cu:285-389): a workgroup takes one (subproblem, transform)
 // job at a time from its XCD's contiguous job range, zeroes an LDS tile
@@ -1025,14 +1069,18 @@ k_spread_sm(int M, SortedPts sp, const cpx<T> *__restrict__ c, cpx<T> *__restric
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int total = sa.counters[1];
-    jobq_run(jq, (long long)total * ntr, [&](long long job) {
-        const SubprobJob J = subprob_job(job, total, sa);
-        if (sizeof(T) == 4 && J.np <= kp.fx32max)
-            spread_sm_job_fx<T, int, DIM, NS, BT>(J, M, sp, c, fw, g, kp, nftot, smem, sa.maxsub);
-        else
-            spread_sm_job_fx<T, long long, DIM, NS, BT>(J, M, sp, c, fw, g, kp, nftot, smem,
-                                                        sa.maxsub);
-    });
+    jobq_run(
+        jq, (long long)total * ntr,
+        [&](long long job) {
+            const SubprobJob J = subprob_job(job, total, sa);
+            if (sizeof(T) == 4 && J.np <= kp.fx32max)
+                spread_sm_job_fx<T, int, DIM, NS, BT>(J, M, sp, c, fw, g, kp, nftot, smem,
+                                                      sa.maxsub);
+            else
+                spread_sm_job_fx<T, long long, DIM, NS, BT>(J, M, sp, c, fw, g, kp, nftot, smem,
+                                                            sa.maxsub);
+        },
+        [&](long long job) { return subprob_pos(job, total, M, sa); }, (long long)M * ntr);
 }
 
 // One member of a plan group (many independent plans of one type, dimension,
@@ -1463,9 +1511,13 @@ k_interp_sm(int M, SortedPts sp, cpx<T> *__restrict__ c, const cpx<T> *__restric
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int total = sa.counters[1];
-    jobq_run(jq, (long long)total * ntr, [&](long long job) {
-        interp_sm_job<T, DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g, kp, nftot, smem);
-    });
+    jobq_run(
+        jq, (long long)total * ntr,
+        [&](long long job) {
+            interp_sm_job<T, DIM, NS>(subprob_job(job, total, sa), M, sp, c, fw, g, kp, nftot,
+                                      smem);
+        },
+        [&](long long job) { return subprob_pos(job, total, M, sa); }, (long long)M * ntr);
 }
 
 // SM interp of a whole plan group in one persistent launch.

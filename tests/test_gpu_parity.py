@@ -279,28 +279,45 @@ def test_type2(torch_dev, case, method, sort):
     assert rel_l2(c, oc) <= tol / 5
 
 
+def _xcd_work_imbalance(binsize, maxsub):
+    """The scheduler's estimate: jobs in bin order split into 8 equal-count
+    ranges, work = points + 8 per job; returns max / mean over the ranges."""
+    jobs = np.concatenate([np.minimum(maxsub, n - maxsub * np.arange(-(-n // maxsub)))
+                           for n in binsize if n > 0])
+    w = jobs + 8
+    n = len(w)
+    rng = [w[n * x // 8:n * (x + 1) // 8].sum() for x in range(8)]
+    return max(rng) / np.mean(rng), n
+
+
 @pytest.mark.parametrize("nufft_type", [1, 2])
 def test_dynamic_job_queues_repeated(torch_dev, nufft_type):
-    """Launches with at least 4 jobs per workgroup take the second half of
-    each XCD's job range from atomic counters, which the last workgroup
-    clears for the next launch.  Large tiles (one workgroup per CU, so a
-    256-workgroup grid) with 64-point subproblems give ~1800 jobs; three
-    executes in a row must all match the oracle (a counter left dirty would
-    skip jobs on the next launch) and each other."""
+    """Launches with at least 4 jobs per workgroup whose XCD ranges carry
+    unequal work (here: sparse uniform bins of few points and two tight
+    clusters of full subproblems) take the second half of each range from
+    atomic counters, which the last workgroup clears for the next launch.
+    Large tiles (one workgroup per CU, a 256-workgroup grid) with 64-point
+    subproblems give ~3000 jobs; three executes in a row must all match the
+    oracle (a counter left dirty would skip jobs on the next launch) and each
+    other."""
     import torch
     from cufinufft_amd import cufinufft
-    dim, nm, M = 3, (32, 32, 32), 100_000
+    dim, nm = 3, (64, 64, 64)
     if nufft_type == 1:
         dtype, tol, bins = np.float32, 1e-6, (16, 16, 8)    # 64-bit tile, 147 KiB
     else:
         dtype, tol, bins = np.float64, 1e-9, (16, 16, 4)    # FP64 tile, 151 KiB
     _flush(f"dynamic job queues type {nufft_type}")
-    pts = G.clustered_pts(M, dim, dtype, seed=51)
+    u = G.uniform_pts(20_000, dim, dtype, seed=51)
+    cl = G.clustered_pts(80_000, dim, dtype, seed=51, ncl=2, sigma=0.1)
+    pts = [np.concatenate([a, b]) for a, b in zip(u, cl)]
+    M = pts[0].size
     opts = dict(gpu_method=2, gpu_maxsubprobsize=64, gpu_binsizex=bins[0],
                 gpu_binsizey=bins[1], gpu_binsizez=bins[2])
     plan = cufinufft(nufft_type, nm[::-1], eps=tol, dtype=dtype, **opts)
     plan.set_pts(*[G.to_dev(p, torch_dev) for p in pts[::-1]])
-    assert plan.info()["nsubprob"] >= 4 * 256
+    imb, njobs = _xcd_work_imbalance(plan.sort_arrays()["binsize"], 64)
+    assert njobs >= 4 * 256 and imb > 1.1
     P = O.Plan(nufft_type, list(nm), tol, single=dtype == np.float32)
     outs = []
     if nufft_type == 1:
