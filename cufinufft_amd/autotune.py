@@ -45,8 +45,9 @@ def candidates(nufft_type, dim):
     else:
         bins = [(1024, 1, 1), (512, 1, 1), (2048, 1, 1)]
     # subproblems of >= 1024 points run 1024-thread workgroups (type 2 also
-    # 512-thread ones from 512 points)
-    subs = (256, 1024, 2048, 4096) if nufft_type == 1 else (256, 512, 1024, 2048, 4096)
+    # 512-thread ones from 512 points); type 1: jobs of up to 536 points keep
+    # the 32-bit fixed-point tile at tol 1e-5, hence 512
+    subs = (256, 512, 1024, 2048, 4096)
     out = []
     for b in bins:
         for msub in subs:
