@@ -160,7 +160,20 @@ def _layout(opts):
     return {k: v for k, v in (opts or {}).items() if k != "gpu_kerevalmeth"}
 
 
-def measured_traffic(workload, kernel_prefix, opts):
+def _kernel_matches(kern, tmpl):
+    """kern: a profiler kernel name 'void cfa::NAME<T, [Acc, ]DIM, NS[, BT]>(...)';
+    tmpl: (NAME, T, DIM, NS)."""
+    name, typ, dim, ns = tmpl
+    head = f"void cfa::{name}<"
+    if not kern.startswith(head):
+        return False
+    args = [x.strip() for x in kern[len(head):].split(">")[0].split(",")]
+    ints = [i for i, x in enumerate(args) if x.isdigit()]
+    return (bool(args) and args[0] == typ and len(ints) >= 2 and
+            int(args[ints[0]]) == dim and int(args[ints[0] + 1]) == ns)
+
+
+def measured_traffic(workload, tmpl, opts):
     """HBM bytes per launch of the kernel for this workload and options, from
     the TRAFFIC records of the committed rocprofv3 summaries in profiles/
     (FETCH_SIZE and WRITE_SIZE runs scaled by a plain-copy calibration), or
@@ -177,10 +190,8 @@ def measured_traffic(workload, kernel_prefix, opts):
             if not line.startswith("TRAFFIC "):
                 continue
             rec = json.loads(line[len("TRAFFIC "):])
-            kern = rec.get("kernel", "")
             if (rec.get("workload") == workload and _layout(rec.get("opts")) == _layout(opts)
-                    and (kern.startswith(kernel_prefix + ">")
-                         or kern.startswith(kernel_prefix + ","))):
+                    and _kernel_matches(rec.get("kernel", ""), tmpl)):
                 return float(rec["bytes"])
     return None
 
@@ -197,13 +208,13 @@ def algorithmic_bytes(w, nf, kind):
     return T * M * 2 * s + M * d * s + 4 * M + T * nftot * 2 * s
 
 
-def run_plan(w, dev, dpts, din, opts, steps, warmup, maxbatch):
+def run_plan(w, dev, dpts, din, opts, steps, warmup, maxbatch, accuracy_first=False):
     import torch
     from cufinufft_amd import cufinufft
     dt = np.dtype(w["dtype"])
     shape_np = tuple(w["N"][::-1])
     plan = cufinufft(w["type"], shape_np, n_trans=w["ntransf"], eps=w["tol"], dtype=dt,
-                     maxbatchsize=maxbatch, **opts)
+                     maxbatchsize=maxbatch, accuracy_first=accuracy_first, **opts)
     # setpts throughput (median of 3)
     plan.set_pts(*dpts[::-1])
     torch.cuda.synchronize()
@@ -443,15 +454,145 @@ def bench_autograd(a, dev):
     }
 
 
+def bench_spreadinterp(a, dev):
+    """Stand-alone spread and interpolation (gpu_spreadinterponly = 1: the
+    caller's array is the fine grid) on the grid sizes and tolerances of the
+    reference's spreader benchmark (test/spreadperf.sh): M = 1e6 uniform
+    points; 1D n = 1e6, 2D n = 1000^2, 3D n = 100^3; tol 1e-12 (FP64) and 1e-5
+    (FP32); methods 1 (GM-sort) and 2 (SM), each where it can run (the FP64
+    ns = 16 SM tile of 3D exceeds the LDS, as in the reference).  Reports NU
+    points per second of one spread or interp pass per case."""
+    import torch
+    from cufinufft_amd import cufinufft
+    M = 1_000_000
+    rng = np.random.default_rng(6)
+    grids = {1: (1_000_000,), 2: (1000, 1000), 3: (100, 100, 100)}
+    cases = []
+    for dim in (1, 2, 3):
+        n = grids[dim]
+        for prec, tol in (("float64", 1e-12), ("float32", 1e-5)):
+            dt = np.dtype(prec)
+            cd = torch.complex64 if dt == np.float32 else torch.complex128
+            P = [torch.from_numpy(rng.uniform(-np.pi, np.pi, M).astype(dt)).to(dev)
+                 for _ in range(dim)]
+            c = torch.from_numpy(cplx_normal(rng, (M,), dt)).to(dev)
+            grid = torch.from_numpy(cplx_normal(rng, tuple(n[::-1]), dt)).to(dev)
+            for kind in (1, 2):
+                for meth in (1, 2):
+                    try:
+                        plan = cufinufft(kind, tuple(n[::-1]), eps=tol, dtype=dt,
+                                         gpu_spreadinterponly=1, gpu_method=meth)
+                    except RuntimeError:
+                        cases.append(dict(dim=dim, prec=prec, op="spread" if kind == 1 else
+                                          "interp", method=meth, status="not plannable"))
+                        continue
+                    plan.set_pts(*P[::-1])
+                    args = (c, grid) if kind == 1 else (torch.empty_like(c), grid)
+                    for _ in range(a.warmup):
+                        plan.execute(*args)
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    for _ in range(a.steps):
+                        plan.execute(*args)
+                    torch.cuda.synchronize()
+                    t = (time.perf_counter() - t0) / a.steps
+                    ns = plan.info()["ns"]
+                    cases.append(dict(dim=dim, prec=prec, op="spread" if kind == 1 else "interp",
+                                      method=meth, ns=ns, ms=t * 1e3, pts_per_s=M / t))
+                    log(f"[bench] {dim}D {prec} {'spread' if kind == 1 else 'interp'} "
+                        f"method {meth}: {t * 1e3:.3f} ms, {M / t:.3e} pts/s")
+                    del plan
+            del P, c, grid
+    head = [x for x in cases if x.get("dim") == 3 and x.get("prec") == "float32" and
+            x.get("op") == "spread" and x.get("method") == 2][0]
+    return {
+        "metric": "NU pts/s per spread (3D FP32 tol 1e-5, SM), spread/interp-only, 1 MI355X",
+        "value": head["pts_per_s"], "unit": "NU pts/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": head["ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "spread/interp only, M=1e6 uniform; 1D n=1e6, 2D 1000^2, "
+                               "3D 100^3; tol 1e-12 FP64 / 1e-5 FP32; methods 1 and 2"},
+        "cases": cases, "roofline": None, "cpu_baseline": None,
+    }
+
+
+def measure(w, wname, dev, dpts, din, sel, ref, a, accuracy_first=False):
+    """Plain plan (the reference's default options and kernel width) and the
+    tuned plan (the autotuner's method / bins / subproblem size, planned
+    accuracy-first when asked), both timed over the same executes and checked
+    against the exact sums ``ref`` at ``sel``."""
+    T = w["ntransf"]
+    dt = np.dtype(w["dtype"])
+    maxbatch = min(T, 8)
+    log(f"[bench] {wname}: plain plan (reference default options)")
+    plain = run_plan(w, dev, dpts, din, {}, a.steps, a.warmup, maxbatch)
+    err_plain = check_accuracy(w, plain["out"], sel, ref)
+    log(f"[bench] {wname}: plain {plain['t_exec'] * 1e3:.3f} ms/exec, rel_l2 {err_plain:.3e}, "
+        f"info {plain['info']}")
+    best, best_opts = plain, {}
+    if a.opts and not accuracy_first:
+        best_opts = json.loads(a.opts)
+        log(f"[bench] {wname}: given options {best_opts}")
+        best = run_plan(w, dev, dpts, din, best_opts, a.steps, a.warmup, maxbatch)
+    elif not a.no_tune:
+        from cufinufft_amd import autotune
+        log(f"[bench] {wname}: autotuning (cufinufft_amd.autotune)"
+            + (", accuracy-first plans" if accuracy_first else ""))
+        best_opts, _ = autotune.tune(w["type"], tuple(w["N"][::-1]), dpts[::-1], w["tol"],
+                                     dtype=dt, n_trans=T, maxbatchsize=maxbatch,
+                                     use_cache=False, log=log, accuracy_first=accuracy_first)
+        log(f"[bench] {wname}: tuned choice {best_opts}; re-timing")
+        best = run_plan(w, dev, dpts, din, best_opts, a.steps, a.warmup, maxbatch,
+                        accuracy_first=accuracy_first)
+        # at equal kernel width the plain options are a candidate too; an
+        # accuracy-first plan is compared at its own (wider) width only
+        if plain["t_exec"] < best["t_exec"] and not accuracy_first:
+            best, best_opts = plain, {}
+    err_best = check_accuracy(w, best["out"], sel, ref)
+    log(f"[bench] {wname}: tuned {best['t_exec'] * 1e3:.3f} ms/exec, rel_l2 {err_best:.3e}")
+    M = w["M"]
+    st = best["stages"]["spread_interp"]
+    kern_ms = st[0] / max(st[1], 1)
+    nbytes = algorithmic_bytes(w, best["info"]["nf"], "interp" if w["type"] == 2 else "spread")
+    kname = ("k_interp_" if w["type"] == 2 else "k_spread_") + \
+        {1: "gm", 2: "sm", 4: "blk"}[best["info"]["method"]]
+    # template arguments <T, [Acc,] DIM, NS[, threads]>: match by name, type and NS
+    ktmpl = (kname, "float" if dt == np.float32 else "double", w["dim"], best["info"]["ns"])
+    out = dict(
+        value=M * T / best["t_exec"], ms=best["t_exec"] * 1e3, rel_l2=err_best,
+        opts=best_opts or "reference defaults", accuracy_first=accuracy_first and bool(best_opts),
+        ns=best["info"]["ns"],
+        plain=dict(value=M * T / plain["t_exec"], ms=plain["t_exec"] * 1e3, rel_l2=err_plain,
+                   method=plain["info"]["method"], ns=plain["info"]["ns"]),
+        setpts=dict(pts_per_s=M / best["t_setpts"], ms=best["t_setpts"] * 1e3),
+        stages_ms_per_exec={k: v[0] / max(a.steps, 1) for k, v in best["stages"].items()},
+        kernel=dict(name=kname, tmpl=ktmpl, ms=kern_ms, algorithmic_bytes=nbytes),
+    )
+    del plain, best
+    return out
+
+
+def roofline_of(r, wname):
+    k = r["kernel"]
+    achieved = k["algorithmic_bytes"] / (k["ms"] * 1e-3) / 1e9
+    opts = r["opts"] if isinstance(r["opts"], dict) else {}
+    return {"bound": "hbm", "kernel": k["name"], "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(wname, k["tmpl"], opts),
+            "algorithmic_bytes": k["algorithmic_bytes"], "kernel_ms": k["ms"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c2",
-                    choices=sorted(WORKLOADS) + ["c4many", "c5"])
+    ap.add_argument("--workload", default="c3",
+                    choices=sorted(WORKLOADS) + ["c4many", "c5", "spread3d"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="c3: skip the FP64 (tol 1e-9) pipeline on the same points")
     ap.add_argument("--cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--opts", default=None,
                     help="JSON dict of cufinufft_opts fields for the tuned plan (skips the sweep)")
@@ -477,8 +618,9 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] copied {a.copy_calib} bytes x3")
         return 0
-    if a.workload in ("c4many", "c5"):
-        res = (bench_group if a.workload == "c4many" else bench_autograd)(a, dev)
+    if a.workload in ("c4many", "c5", "spread3d"):
+        fn = {"c4many": bench_group, "c5": bench_autograd, "spread3d": bench_spreadinterp}
+        res = fn[a.workload](a, dev)
         print(json.dumps(res), flush=True)
         return 0
     w = WORKLOADS[a.workload]
@@ -504,69 +646,60 @@ def main():
         log(f"[bench] kernel-only: {r['t_exec'] * 1e3:.3f} ms/exec, setpts "
             f"{r['t_setpts'] * 1e3:.3f} ms, stages {r['stages']}")
         return 0
-    log("[bench] plain plan (reference default options)")
-    plain = run_plan(w, dev, dpts, din, {}, a.steps, a.warmup, maxbatch)
     log("[bench] exact FP64 reference at 1000 sampled outputs")
     sel, ref = direct_reference(w, pts, host_in, dpts, din, np.random.default_rng(99))
-    err_plain = check_accuracy(w, plain["out"], sel, ref)
-    log(f"[bench] plain: {plain['t_exec'] * 1e3:.3f} ms/exec, rel_l2 {err_plain:.3e}, "
-        f"info {plain['info']}")
-    best, best_opts = plain, {}
-    tried = []
-    if a.opts:
-        best_opts = json.loads(a.opts)
-        log(f"[bench] given options {best_opts}")
-        best = run_plan(w, dev, dpts, din, best_opts, a.steps, a.warmup, maxbatch)
-    elif not a.no_tune:
-        from cufinufft_amd import autotune
-        log("[bench] autotuning (cufinufft_amd.autotune)")
-        best_opts, tried = autotune.tune(w["type"], tuple(w["N"][::-1]), dpts[::-1], w["tol"],
-                                         dtype=dt, n_trans=T, maxbatchsize=maxbatch,
-                                         use_cache=False, log=log)
-        log(f"[bench] tuned choice {best_opts}; re-timing")
-        best = run_plan(w, dev, dpts, din, best_opts, a.steps, a.warmup, maxbatch)
-        if plain["t_exec"] < best["t_exec"]:
-            best, best_opts = plain, {}
-    err_best = check_accuracy(w, best["out"], sel, ref)
+    fp64_twin = a.workload == "c3" and not a.no_fp64
+    # a workload at tol < 1e-7 is an FP64 pipeline: its tuned plan is planned
+    # accuracy-first (the reference's width rule misses tol in 2D / 3D)
+    r = measure(w, a.workload, dev, dpts, din, sel, ref, a,
+                accuracy_first=w["dtype"] == "float64" and w["tol"] < 1e-7)
     M = w["M"]
-    value = M * T / best["t_exec"]
-    st = best["stages"]["spread_interp"]
-    kern_ms = st[0] / max(st[1], 1)
-    nbytes = algorithmic_bytes(w, best["info"]["nf"], "interp" if w["type"] == 2 else "spread")
-    achieved = nbytes / (kern_ms * 1e-3) / 1e9
-    kname = ("k_interp_" if w["type"] == 2 else "k_spread_") + \
-        {1: "gm", 2: "sm", 4: "blk"}[best["info"]["method"]]
-    # template arguments <T, DIM, NS[, threads]>: match up to NS
-    ktmpl = f"void cfa::{kname}<{'float' if dt == np.float32 else 'double'}, {w['dim']}, " \
-            f"{best['info']['ns']}"
-    traffic = measured_traffic(a.workload, ktmpl, best_opts)
     res = {
         "metric": "NU pts/s per execute at rel-l2 <= tol: tuned/batched vs plain, 1 MI355X",
-        "value": value,
+        "value": r["value"],
         "unit": "NU pts/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": best["t_exec"] * 1e3,
+        "ms_per_step": r["ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if dt == np.float32 else "f64",
         "data": "synthetic",
-        "config": {"workload": w["desc"], "type": w["type"], "dim": w["dim"],
-                   "n_modes": list(w["N"]), "M": M, "ntransf": T, "tol": w["tol"],
-                   "tuned_opts": best_opts or "reference defaults"},
-        "rel_l2": {"tuned": err_best, "plain": err_plain, "tol": w["tol"]},
-        "plain": {"value": M * T / plain["t_exec"], "ms_per_step": plain["t_exec"] * 1e3,
-                  "method": plain["info"]["method"]},
-        "setpts": {"pts_per_s": M / best["t_setpts"], "ms": best["t_setpts"] * 1e3},
-        "stages_ms_per_exec": {k: v[0] / max(a.steps, 1) for k, v in best["stages"].items()},
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "algorithmic_bytes": nbytes, "kernel_ms": kern_ms},
+        "config": {"workload": w["desc"] + (" -- with the FP64 tol-1e-9 pipeline on the same "
+                                            "points (mixed-precision form)" if fp64_twin else ""),
+                   "type": w["type"], "dim": w["dim"], "n_modes": list(w["N"]), "M": M,
+                   "ntransf": T, "tol": w["tol"], "tuned_opts": r["opts"]},
+        "rel_l2": {"tuned": r["rel_l2"], "plain": r["plain"]["rel_l2"], "tol": w["tol"]},
+        "plain": {"value": r["plain"]["value"], "ms_per_step": r["plain"]["ms"],
+                  "method": r["plain"]["method"], "ns": r["plain"]["ns"]},
+        "tuned_over_plain": r["value"] / r["plain"]["value"],
+        "setpts": r["setpts"],
+        "stages_ms_per_exec": r["stages_ms_per_exec"],
+        "roofline": roofline_of(r, a.workload),
         "cpu_baseline": None,
     }
-    del plain, best
+    if fp64_twin:
+        # the same points and strengths in FP64 (the FP32 values, exactly),
+        # so one exact reference serves both pipelines
+        w64 = dict(WORKLOADS["c3d"])
+        del din, dpts
+        torch.cuda.synchronize()
+        dpts64 = [torch.from_numpy(p.astype(np.float64)).to(dev) for p in pts]
+        din64 = torch.from_numpy(host_in.astype(np.complex128)).to(dev)
+        torch.cuda.synchronize()
+        r64 = measure(w64, "c3d", dev, dpts64, din64, sel, ref, a, accuracy_first=True)
+        res["fp64"] = {"workload": w64["desc"], "tol": w64["tol"], "value": r64["value"],
+                       "ms_per_step": r64["ms"], "rel_l2": r64["rel_l2"],
+                       "tuned_opts": r64["opts"], "accuracy_first": r64["accuracy_first"],
+                       "ns": r64["ns"], "plain": r64["plain"], "setpts": r64["setpts"],
+                       "stages_ms_per_exec": r64["stages_ms_per_exec"],
+                       "roofline": roofline_of(r64, "c3d")}
+        res["rel_l2"] = {"fp32": res["rel_l2"],
+                         "fp64": {"tuned": r64["rel_l2"], "plain": r64["plain"]["rel_l2"],
+                                  "tol": w64["tol"]}}
+        del dpts64, din64
     torch.cuda.synchronize()
     if not a.no_cpu:
         log("[bench] CPU baseline")

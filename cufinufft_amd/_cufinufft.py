@@ -83,6 +83,18 @@ _make_planf.argtypes = [c_int, c_int, c_int_p, c_int, c_int, c_float, c_int,
                         CufinufftPlanf_p, NufftOpts_p]
 _make_planf.restype = c_int
 
+_make_plan_ext = lib.cufinufft_makeplan_ext
+_make_plan_ext.argtypes = [c_int, c_int, c_int_p, c_int, c_int, c_double, c_int,
+                           CufinufftPlan_p, NufftOpts_p, c_int]
+_make_plan_ext.restype = c_int
+
+_make_planf_ext = lib.cufinufftf_makeplan_ext
+_make_planf_ext.argtypes = [c_int, c_int, c_int_p, c_int, c_int, c_float, c_int,
+                            CufinufftPlanf_p, NufftOpts_p, c_int]
+_make_planf_ext.restype = c_int
+
+ACCURACY_FIRST = 1
+
 _set_pts = lib.cufinufft_setpts
 _set_pts.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_int, c_double_p,
                      c_double_p, c_double_p, c_void_p]
@@ -184,6 +196,7 @@ STAGES = ("sort", "grid_zero", "spread_interp", "fft", "deconv_amplify")
 EXPORTED = [
     "cufinufft_default_opts", "cufinufftf_default_opts",
     "cufinufft_makeplan", "cufinufftf_makeplan",
+    "cufinufft_makeplan_ext", "cufinufftf_makeplan_ext",
     "cufinufft_setpts", "cufinufftf_setpts",
     "cufinufft_execute", "cufinufftf_execute",
     "cufinufft_destroy", "cufinufftf_destroy",

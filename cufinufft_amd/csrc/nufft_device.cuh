@@ -14,6 +14,10 @@
 
 namespace cfa {
 
+// Index arithmetic shared by the kernels and the host-side index checks
+// (tests/cpu_index): compiled for both sides.
+#define CFA_HD __host__ __device__ __forceinline__
+
 template <typename T>
 struct alignas(2 * sizeof(T)) cpx {
     T x, y;
@@ -40,9 +44,9 @@ struct KerParams {
     // holding the kernel divided by e^beta (peak 1).
     const T *hc;
     int hdeg;
-    // FP32 SM spread: 1 sizes the LDS tile for 64-bit fixed point, 0 for
-    // 32-bit (chosen per plan from tol and the subproblem size); jobs of at
-    // most fx32max points accumulate in 32 bits either way, larger ones in 64
+    // FP32 fixed-point spread tiles: fx32max is the largest number of points
+    // one 32-bit scale may cover at the plan's tolerance; fxwide (method 4)
+    // selects 64-bit block tiles when its subproblems exceed it
     int fxwide;
     int fx32max;
 };
@@ -101,7 +105,7 @@ __device__ __forceinline__ JobRange xcd_jobs(long long njobs)
 // plan's stream, or one group's).
 static_assert(kJobqWords == kNumXcd + 1, "job queue: a counter per XCD range + a done count");
 
-__device__ __forceinline__ long long jobq_static(long long njobs, int x, int per)
+CFA_HD long long jobq_static(long long njobs, int x, int per)
 {
     const long long n = njobs * (x + 1) / kNumXcd - njobs * x / kNumXcd;
     return n / 2 / per * per;
@@ -230,13 +234,13 @@ struct SubprobArgs {
 // double (one fused multiply-add, as nvcc contracts it by default), times N in
 // double.  Returned unrounded: the caller rounds to T for the bin index (the
 // reference's value, bit for bit) and keeps the double for tap offsets.
-__device__ __forceinline__ double rescale_exact(float x, int n)
+CFA_HD double rescale_exact(float x, int n)
 {
     const float pi = 3.14159265358979323846f;  // (float)M_PI
     const double off = x < -pi ? 1.5 : (x >= pi ? -0.5 : 0.5);
     return __builtin_fma((double)x, kInv2Pi, off) * (double)n;
 }
-__device__ __forceinline__ double rescale_exact(double x, int n)
+CFA_HD double rescale_exact(double x, int n)
 {
     const double pi = 3.14159265358979323846;
     const double off = x < -pi ? 1.5 : (x >= pi ? -0.5 : 0.5);
@@ -248,7 +252,7 @@ __device__ __forceinline__ double rescale_exact(double x, int n)
 // 113-118).  Written so that NaN or far out-of-range input (which the
 // reference indexes out of bounds with) lands in a valid bin instead.
 template <typename T>
-__device__ __forceinline__ int bin1(double v, int bs, int nb)
+CFA_HD int bin1(double v, int bs, int nb)
 {
     const T xt = (T)v;
     const T q = floor(xt / (T)bs);
@@ -282,7 +286,7 @@ __device__ __forceinline__ double es_eval(double x, const KerParams<double> &k)
 
 // Position used for tap offsets: the unrounded rescale, clamped to
 // [-1, n+1] so that invalid input cannot index outside one period of wrap.
-__device__ __forceinline__ double tap_pos(double v, int n)
+CFA_HD double tap_pos(double v, int n)
 {
     return fmin(fmax(v, -1.0), (double)n + 1.0);
 }
@@ -338,11 +342,11 @@ __device__ __forceinline__ int taps_local(double v, int o, int tw, const KerPara
     return l;
 }
 
-__device__ __forceinline__ int wrap1(int i, int n)
+CFA_HD int wrap1(int i, int n)
 {
     return i < 0 ? i + n : (i >= n ? i - n : i);
 }
-__device__ __forceinline__ int wrapmod(int i, int n)
+CFA_HD int wrapmod(int i, int n)
 {
     i %= n;
     return i < 0 ? i + n : i;
@@ -374,7 +378,7 @@ __device__ __forceinline__ void ladd(double *p, double v)
 // wave touch neighbouring LDS cells (or the same cell, which the LDS
 // broadcasts); the per-bin counts and starts are unchanged.
 template <typename T>
-__device__ __forceinline__ void bin_and_cell(double v, int bs, int nb, int &b, int &lc)
+CFA_HD void bin_and_cell(double v, int bs, int nb, int &b, int &lc)
 {
     const T xt = (T)v;
     const T q = floor(xt / (T)bs);
@@ -650,130 +654,6 @@ k_interp_gm(int M, SortedPts sp, cpx<T> *__restrict__ c, const cpx<T> *__restric
     }
 }
 
-// ES kernel and its derivative with respect to the point's grid coordinate
-// xr, at tap offset t = l - xr: phi(t) and d phi(l - xr)/d xr = -phi'(t) =
-// phi(t) beta c t / sqrt(1 - c t^2).  The derivative is integrable but
-// unbounded at |t| -> ns/2, where phi itself has fallen to exp(0) = 1, i.e.
-// exp(-beta) of its peak; the square root is floored to keep it finite.
-__device__ __forceinline__ void es_eval_grad(float t, const KerParams<float> &k, float &v,
-                                             float &dv)
-{
-    const float a = 1.0f - k.c * t * t;
-    const bool in = fabsf(t) < k.hw && a > 0.0f;
-    const float s = sqrtf(fmaxf(a, 1e-12f));
-    v = in ? expf(k.beta * s) : 0.0f;
-    dv = in ? v * k.beta * k.c * t / s : 0.0f;
-}
-__device__ __forceinline__ void es_eval_grad(double t, const KerParams<double> &k, double &v,
-                                             double &dv)
-{
-    const double a = 1.0 - k.c * t * t;
-    const bool in = fabs(t) < k.hw && a > 0.0;
-    const double s = sqrt(fmax(a, 1e-30));
-    v = in ? exp(k.beta * s) : 0.0;
-    dv = in ? v * k.beta * k.c * t / s : 0.0;
-}
-
-template <typename T, int NS>
-__device__ __forceinline__ int taps_global_grad(double v, const KerParams<T> &kp, T *ker, T *dker)
-{
-    const int start = (int)ceil(v - 0.5 * NS);
-    const T x1 = (T)((double)start - v);
-#pragma unroll
-    for (int i = 0; i < NS; ++i) es_eval_grad(x1 + (T)i, kp, ker[i], dker[i]);
-    return start;
-}
-
-// Interp with gradient (type-2 plans, the differentiable form): one thread
-// per sorted point gathers the ns^DIM taps once and accumulates the value
-// c_j = sum_l fw_l prod_d phi_d and its DIM partial derivatives with respect
-// to the point's grid coordinates (one factor replaced by d phi / d xr_d),
-// scaled by d xr / d x = nf_d / (2 pi) into derivatives with respect to x_d.
-// dc holds DIM planes of ntr*M complex values.
-template <typename T, int DIM, int NS>
-__global__ void __launch_bounds__(256)
-k_interp_grad(int M, SortedPts sp, cpx<T> *__restrict__ c, cpx<T> *__restrict__ dc,
-              const cpx<T> *__restrict__ fw, BinGeom g, KerParams<T> kp, int ntr,
-              long long nftot, long long dc_plane)
-{
-    const int n1 = g.nf[0], n2 = g.nf[1], n3 = g.nf[2];
-    const T sc1 = (T)(n1 * kInv2Pi), sc2 = (T)(n2 * kInv2Pi), sc3 = (T)(n3 * kInv2Pi);
-    const JobRange jr = xcd_jobs((M + kPtChunk - 1) / kPtChunk);
-    for (long long ch = jr.begin; ch < jr.end; ch += jr.step) {
-        const int i = (int)(ch * kPtChunk) + (int)threadIdx.x;
-        if (i >= M) continue;
-        const int j = sp.idx[i];
-        T k1[NS], k2[NS], k3[NS], d1[NS], d2[NS], d3[NS];
-        const int s1 = taps_global_grad<T, NS>(sp.pos[i], kp, k1, d1);
-        int s2 = 0, s3 = 0;
-        if (DIM > 1) s2 = taps_global_grad<T, NS>(sp.pos[sp.stride + i], kp, k2, d2);
-        if (DIM > 2) s3 = taps_global_grad<T, NS>(sp.pos[2 * sp.stride + i], kp, k3, d3);
-        int ix[NS];
-#pragma unroll
-        for (int a = 0; a < NS; ++a) ix[a] = wrap1(s1 + a, n1);
-        for (int t = 0; t < ntr; ++t) {
-            const cpx<T> *f = fw + (long long)t * nftot;
-            // value, d/dx1, d/dx2, d/dx3 (re, im)
-            T vr = 0, vi = 0, gxr = 0, gxi = 0, gyr = 0, gyi = 0, gzr = 0, gzi = 0;
-#pragma unroll
-            for (int dz = 0; dz < (DIM > 2 ? NS : 1); ++dz) {
-                const int iz = DIM > 2 ? wrap1(s3 + dz, n3) : 0;
-                const T kz = DIM > 2 ? k3[dz] : (T)1;
-                const T dkz = DIM > 2 ? d3[dz] : (T)0;
-                T pr = 0, pi = 0, pxr = 0, pxi = 0, pyr = 0, pyi = 0;  // this z-plane
-#pragma unroll
-                for (int dy = 0; dy < (DIM > 1 ? NS : 1); ++dy) {
-                    const int iy = DIM > 1 ? wrap1(s2 + dy, n2) : 0;
-                    const T ky = DIM > 1 ? k2[dy] : (T)1;
-                    const T dky = DIM > 1 ? d2[dy] : (T)0;
-                    const cpx<T> *row = f + ((long long)iz * n2 + iy) * n1;
-                    T rr = 0, ri = 0, rxr = 0, rxi = 0;
-#pragma unroll
-                    for (int a = 0; a < NS; ++a) {
-                        const cpx<T> v = row[ix[a]];
-                        rr += v.x * k1[a];
-                        ri += v.y * k1[a];
-                        rxr += v.x * d1[a];
-                        rxi += v.y * d1[a];
-                    }
-                    pr += rr * ky;
-                    pi += ri * ky;
-                    pxr += rxr * ky;
-                    pxi += rxi * ky;
-                    pyr += rr * dky;
-                    pyi += ri * dky;
-                }
-                vr += pr * kz;
-                vi += pi * kz;
-                gxr += pxr * kz;
-                gxi += pxi * kz;
-                gyr += pyr * kz;
-                gyi += pyi * kz;
-                gzr += pr * dkz;
-                gzi += pi * dkz;
-            }
-            const long long o = (long long)t * M + j;
-            cpx<T> w;
-            w.x = vr;
-            w.y = vi;
-            c[o] = w;
-            w.x = gxr * sc1;
-            w.y = gxi * sc1;
-            dc[o] = w;
-            if (DIM > 1) {
-                w.x = gyr * sc2;
-                w.y = gyi * sc2;
-                dc[dc_plane + o] = w;
-            }
-            if (DIM > 2) {
-                w.x = gzr * sc3;
-                w.y = gzi * sc3;
-                dc[2 * dc_plane + o] = w;
-            }
-        }
-    }
-}
-
 // Tile of one subproblem: the bin's cells plus a halo of P = ceil(ns/2) on
 // each side; cell 0 of the tile is global index o[d] = bin*bs - P.
 template <int DIM, int NS>
@@ -782,7 +662,7 @@ struct Tile {
     int tw[3];
     int o[3];
     int n;
-    __device__ __forceinline__ Tile(const BinGeom &g, int b)
+    CFA_HD Tile(const BinGeom &g, int b)
     {
         const int bx = b % g.nb[0];
         const int by = (b / g.nb[0]) % g.nb[1];
@@ -796,7 +676,7 @@ struct Tile {
         n = tw[0] * tw[1] * tw[2];
     }
     // fine-grid offset of tile cell q (periodic wrap of every axis)
-    __device__ __forceinline__ long long grid_offset(int q, const BinGeom &g) const
+    CFA_HD long long grid_offset(int q, const BinGeom &g) const
     {
         const int qx = q % tw[0];
         const int qy = (q / tw[0]) % tw[1];
@@ -808,7 +688,7 @@ struct Tile {
     }
     // Every tile coordinate o + q of every axis lies in [-nf, 2 nf), so one
     // compare each way wraps it (false only for tiles wider than the grid).
-    __device__ __forceinline__ bool walkable(const BinGeom &g) const
+    CFA_HD bool walkable(const BinGeom &g) const
     {
         bool ok = o[0] >= -g.nf[0] && o[0] + tw[0] <= 2 * g.nf[0];
         if (DIM > 1) ok = ok && o[1] >= -g.nf[1] && o[1] + tw[1] <= 2 * g.nf[1];
@@ -823,7 +703,7 @@ struct Tile {
 // integer division (six per cell in grid_offset, which dominated the index
 // arithmetic of the tile loads and flushes).
 template <int DIM, int NS, class F>
-__device__ __forceinline__ void for_tile_cells(const Tile<DIM, NS> &tl, const BinGeom &g, int q0,
+CFA_HD void for_tile_cells(const Tile<DIM, NS> &tl, const BinGeom &g, int q0,
                                                int step, F &&f)
 {
     if (!tl.walkable(g)) {
@@ -868,7 +748,7 @@ struct SubprobJob {
     int t, b, p0, np;
 };
 
-__device__ __forceinline__ SubprobJob subprob_job(long long job, int total, const SubprobArgs &sa)
+CFA_HD SubprobJob subprob_job(long long job, int total, const SubprobArgs &sa)
 {
     SubprobJob J;
     J.t = (int)(job / total);
@@ -889,37 +769,6 @@ __device__ __forceinline__ long long subprob_pos(long long job, int total, int M
     const int s = (int)(job - (long long)t * total);
     const int b = sa.s2b[s];
     return (long long)t * M + sa.binstart[b] + (long long)(s - sa.substart[b]) * sa.maxsub;
-}
-
-// SM spread (Spread_{2,3}d_Subprob, spreadinterp2d.cu:152-231,
-// spreadinterp3d.cu:285-389): a workgroup takes one (subproblem, transform)
-// job at a time from its XCD's contiguous job range, zeroes an LDS tile
-// (separate re / im planes, so consecutive x taps hit consecutive banks),
-// accumulates every tap of the subproblem's points with LDS atomics, then
-// flushes the non-zero tile cells, row by row (coalesced), to the fine grid
-// with global float atomics (the halo overlaps neighbouring bins).  The job
-// count comes from device memory, so setpts never copies it back.
-//
-// Kernel values normalised by their peak: phi(t) / exp(beta) in (0, 1].
-template <typename T, int NS>
-__device__ __forceinline__ int taps_local_unit(double v, int o, int tw, const KerParams<T> &kp,
-                                               T *ker)
-{
-    const int l0 = (int)ceil(v - 0.5 * NS) - o;
-    const int l = l0 < 0 ? 0 : (l0 > tw - NS ? tw - NS : l0);
-    if (kp.hdeg > 0 && l == l0) {
-        horner_taps<T, NS>((T)(2.0 * ((double)(o + l) - v) + (NS - 1)),
-                           kp.hc + (kp.hdeg + 1) * NS, kp.hdeg, ker);
-        return l;
-    }
-    const T x1 = (T)((double)(o + l) - v);
-#pragma unroll
-    for (int i = 0; i < NS; ++i) {
-        const T t = x1 + (T)i;
-        const T a = fmax(T(1) - kp.c * t * t, T(0));
-        ker[i] = fabs(t) < kp.hw ? exp(kp.beta * (sqrt(a) - T(1))) : T(0);
-    }
-    return l;
 }
 
 // Sum over the NW waves of the workgroup (returned to all); scratch holds NW
@@ -944,14 +793,6 @@ __device__ __forceinline__ T block_sum256(T v, T *scratch)
     return block_sum<T, 4>(v, scratch);
 }
 
-// LDS bytes of the fixed-point SM spread: 32- or 64-bit re / im per tile
-// cell, the job's strengths staged as complex T (elem = sizeof(T)), and 16
-// values of reduction scratch (one per wave of a 1024-thread workgroup).
-__host__ __device__ inline size_t fx_spread_lds(size_t cells, int maxsub, int wide, int elem)
-{
-    return cells * (wide ? 16 : 8) + (size_t)maxsub * 2 * elem + 16 * (size_t)elem;
-}
-
 // Fixed-point accumulator widths: headroom bits of the tile scale, the
 // round-to-nearest conversion and the LDS atomic (ds_add_u32 / ds_add_u64;
 // two's-complement sums are exact modulo 2^bits, and the scale keeps every
@@ -972,96 +813,231 @@ template <> struct FxAcc<long long> {
     }
 };
 
-// SM spread job with fixed-point LDS accumulation, for both plan precisions
-// (T = float with Acc = int or long long, T = double with Acc = long long).
-// On gfx950 ds_add_u32 sustains 8-14 lane-ops per CU clock, ds_add_u64 4.9,
-// against 2.5 for ds_add_f64 and 0.33 for ds_add_f32 (measured).  Exactness
-// by construction: with the kernel normalised to phi/exp(beta) <= 1 per axis,
-// no tile cell can exceed sum_p (|re c_p| + |im c_p|) =: A over the job's
-// points, so the job scales contributions by the power of two
-// S = 2^kBits / A (rounded down), inside the signed range; each contribution
-// is rounded to nearest, an absolute error of at most A * 2^-(kBits+1) per
-// tap.  With 32 bits that error grows with the points per job (A); FP32 plans
-// pick 64 bits when it would matter at their tolerance, and FP64 plans always
-// use 64 bits (A * 2^-63 per tap: the rounding level of FP64 sums of size A).
-// The width is chosen per job from its point count (A grows with it), so the
-// many small jobs of a plan whose largest ones need 64 bits still take the
-// faster 32-bit atomics.
-// The flush multiplies by exp(DIM beta) / S in FP64.
-template <typename T, typename Acc, int DIM, int NS, int BT>
-__device__ __forceinline__ void spread_sm_job_fx(const SubprobJob &J, int M, const SortedPts &sp,
-                                                 const cpx<T> *__restrict__ c,
-                                                 cpx<T> *__restrict__ fw, const BinGeom &g,
-                                                 const KerParams<T> &kp, long long nftot,
-                                                 unsigned char *smem, int maxsub)
+// Kernel values phi(x1 + i) / e^beta (peak 1) of the ns taps of one axis from
+// the offset x1 = start - v of the first tap (the polynomial table when it
+// was fitted and x1 lies in its interval [-ns/2, -ns/2 + 1), i.e. the tap
+// window was not clamped to the tile).
+template <typename T, int NS>
+__device__ __forceinline__ void taps_from_offset(T x1, const KerParams<T> &kp, T *ker)
 {
+    if (kp.hdeg > 0 && x1 >= -kp.hw && x1 < -kp.hw + T(1)) {
+        horner_taps<T, NS>(T(2) * x1 + T(NS - 1), kp.hc + (kp.hdeg + 1) * NS, kp.hdeg, ker);
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        const T t = x1 + (T)i;
+        const T a = fmax(T(1) - kp.c * t * t, T(0));
+        ker[i] = fabs(t) < kp.hw ? exp(kp.beta * (sqrt(a) - T(1))) : T(0);
+    }
+}
+
+// Tap window of a point at grid coordinate v along one axis of a tile whose
+// cell 0 is global index o and which is tw cells wide: the first tap
+// ceil(v - ns/2), made tile-local and clamped into [0, tw - ns] (a tap
+// outside the tile is possible only within rounding of the kernel edge,
+// where the kernel is 0); x1 receives the offset (o + l) - v of the window's
+// first tap, at which the kernel values are evaluated.
+template <int NS>
+CFA_HD int tile_window(double v, int o, int tw, double &x1)
+{
+    const int l0 = (int)ceil(v - 0.5 * NS) - o;
+    const int l = l0 < 0 ? 0 : (l0 > tw - NS ? tw - NS : l0);
+    x1 = (double)(o + l) - v;
+    return l;
+}
+
+// Slot of the rank-th point of residue key in the round-robin order over nk
+// residues with counts cnt: the points of earlier rounds, then those of
+// earlier residues in round rank.
+CFA_HD int rr_slot(const int *cnt, int nk, int key, int rank)
+{
+    int slot = 0;
+    for (int k = 0; k < nk; ++k) {
+        const int ck = cnt[k];
+        slot += (ck < rank ? ck : rank) + (k < key && ck > rank ? 1 : 0);
+    }
+    return slot;
+}
+
+// SM spread of one (subproblem, transform) job (Spread_{2,3}d_Subprob,
+// spreadinterp2d.cu:152-231, spreadinterp3d.cu:285-389): the workgroup
+// accumulates every tap of the subproblem's points in an LDS tile of the
+// bin plus a halo of ceil(ns/2), then flushes the non-zero cells to the fine
+// grid with global float atomics, row by row (the halo overlaps neighbouring
+// bins).  Two measured properties of the gfx950 LDS shape it.
+//
+// Integer atomics.  ds_add_u32 runs at the LDS store rate, ds_add_f32 far
+// below it, so the tile holds fixed-point re / im planes.  With the kernel
+// normalised to phi / e^beta <= 1 per axis, no cell can exceed
+// A = sum_p (|re c_p| + |im c_p|) over the points accumulated, so the
+// contributions are scaled by the power of two S = 2^(kBits - e), A < 2^e,
+// and rounded to nearest: exact sums, an error of at most A 2^-(kBits+1) per
+// tap.  64-bit cells (Acc = long long; FP64 plans, and FP32 plans at tight
+// tolerances) take one scale for the whole job.  32-bit cells keep that
+// error within the plan's budget only for a bounded A, so the job runs in
+// chunks of BT points, at most the plan's 32-bit bound (536 at tol 1e-5):
+// each chunk takes its own scale, and between chunks the tile is drained
+// into FP32 planes (plain LDS reads and writes, each thread on its own
+// cells).  The flush multiplies by e^(DIM beta) / S in FP64 (in FP32 the
+// rounding of DIM beta ~ 50 alone would scale the tile by up to 2e-6).
+//
+// Bank-balanced order.  Tap (dz, dy, a) of a point whose first tap is tile
+// cell s lands on s + (dz tw1 + dy) tw0 + a, an offset common to all points,
+// so two lanes of one wave-wide atomic collide on an LDS bank exactly when
+// their start cells agree modulo the bank count, and then they collide at
+// every tap (random start cells: 8-14 lane-ops per clock instead of the
+// full rate).  Each chunk is therefore staged in LDS in round-robin order of
+// the residue s mod NK (NK = 64 for 4-byte cells, 32 for 8-byte cells):
+// round r holds the r-th point of every residue that has one, in residue
+// order, so the lanes of a wave carry distinct residues except where rounds
+// run short.  Staging also evaluates the tap windows once (start cell and
+// the first-tap offset of each axis), so the tap pass reads LDS only.
+template <typename T, typename Acc, int DIM, int NS, int BT>
+__device__ __forceinline__ void spread_sm_job(const SubprobJob &J, int M, const SortedPts &sp,
+                                              const cpx<T> *__restrict__ c,
+                                              cpx<T> *__restrict__ fw, const BinGeom &g,
+                                              const KerParams<T> &kp, long long nftot,
+                                              unsigned char *smem)
+{
+    constexpr bool DRAIN = sizeof(Acc) == 4;
+    constexpr int NK = sizeof(Acc) == 4 ? 64 : 32;
     const Tile<DIM, NS> tl(g, J.b);
+    const int n = tl.n;
     Acc *sre = reinterpret_cast<Acc *>(smem);
-    Acc *sim = sre + tl.n;
-    cpx<T> *cst = reinterpret_cast<cpx<T> *>(sim + tl.n);  // 2*sizeof(Acc)*tl.n bytes in: aligned
-    T *red = reinterpret_cast<T *>(cst + maxsub);
-    for (int q = threadIdx.x; q < tl.n; q += BT) {
-        sre[q] = 0;
-        sim[q] = 0;
+    Acc *sim = sre + n;
+    int *rs = reinterpret_cast<int *>(sim + n);
+    T *rx = reinterpret_cast<T *>(rs + BT);  // DIM planes of first-tap offsets
+    T *rcr = rx + DIM * BT;
+    T *rci = rcr + BT;
+    int *cnt = reinterpret_cast<int *>(rci + BT);
+    T *red = reinterpret_cast<T *>(cnt + 64);
+    // drain planes: present in the allocation only when jobs can exceed one
+    // chunk (J.np > BT), and touched only then
+    float *dre = reinterpret_cast<float *>(red + BT / 64);
+    float *dimg = dre + n;
+    const int tid = threadIdx.x;
+    const int nch = (J.np + BT - 1) / BT;
+    for (int p = tid; p < n; p += BT) {
+        sre[p] = 0;
+        sim[p] = 0;
+        if (DRAIN && nch > 1) {
+            dre[p] = 0.0f;
+            dimg[p] = 0.0f;
+        }
     }
-    T part = 0;
-    for (int q = threadIdx.x; q < J.np; q += BT) {
-        const cpx<T> cv = c[(long long)J.t * M + sp.idx[J.p0 + q]];
-        cst[q] = cv;
-        part += fabs(cv.x) + fabs(cv.y);
+    const cpx<T> *cj = c + (long long)J.t * M;
+    int sh = 0;
+    if (!DRAIN) {  // one scale for the whole job: its strength mass first
+        T part = 0;
+        for (int p = tid; p < J.np; p += BT) {
+            const cpx<T> v = cj[sp.idx[J.p0 + p]];
+            part += fabs(v.x) + fabs(v.y);
+        }
+        const T A = block_sum<T, BT / 64>(part, red);
+        int ex = 0;
+        frexp(A, &ex);
+        sh = A > T(0) ? FxAcc<Acc>::kBits - ex : 0;
     }
-    const T A = block_sum<T, BT / 64>(part, red);  // also orders the stores above
-    int ex = 0;
-    frexp(A, &ex);                        // A < 2^ex
-    const int sh = A > T(0) ? FxAcc<Acc>::kBits - ex : 0;  // S = 2^sh, applied with ldexp
-    for (int q = threadIdx.x; q < J.np; q += BT) {
-        const int i = J.p0 + q;
-        T k1[NS], k2[NS], k3[NS];
-        const int l1 = taps_local_unit<T, NS>(sp.pos[i], tl.o[0], tl.tw[0], kp, k1);
-        int l2 = 0, l3 = 0;
-        if (DIM > 1)
-            l2 = taps_local_unit<T, NS>(sp.pos[sp.stride + i], tl.o[1], tl.tw[1], kp, k2);
-        if (DIM > 2)
-            l3 = taps_local_unit<T, NS>(sp.pos[2 * sp.stride + i], tl.o[2], tl.tw[2], kp, k3);
-        const cpx<T> cv = cst[q];
-        const T sr = ldexp(cv.x, sh), si = ldexp(cv.y, sh);
+    const double eb = exp((double)DIM * (double)kp.beta);
+    for (int ch = 0; ch < nch; ++ch) {
+        const int nc = min(BT, J.np - ch * BT);
+        if (tid < NK) cnt[tid] = 0;
+        __syncthreads();  // orders the zeroing / the previous drain and tap pass
+        int key = 0, rank = 0, s = 0;
+        T x1[3] = {T(0), T(0), T(0)};
+        cpx<T> cv;
+        cv.x = T(0);
+        cv.y = T(0);
+        if (tid < nc) {
+            const int i = J.p0 + ch * BT + tid;
+            cv = cj[sp.idx[i]];
+            int l[3] = {0, 0, 0};
 #pragma unroll
-        for (int dz = 0; dz < (DIM > 2 ? NS : 1); ++dz) {
-            const T kz = DIM > 2 ? k3[dz] : T(1);
+            for (int d = 0; d < DIM; ++d) {
+                double off;
+                l[d] = tile_window<NS>(sp.pos[d * sp.stride + i], tl.o[d], tl.tw[d], off);
+                x1[d] = (T)off;
+            }
+            s = l[0] + tl.tw[0] * (l[1] + tl.tw[1] * l[2]);
+            key = s & (NK - 1);
+            rank = atomicAdd(&cnt[key], 1);
+        }
+        if (DRAIN) {  // per-chunk scale; the reduction also completes the counts
+            const T A = block_sum<T, BT / 64>(tid < nc ? fabs(cv.x) + fabs(cv.y) : T(0), red);
+            int ex = 0;
+            frexp(A, &ex);
+            sh = A > T(0) ? FxAcc<Acc>::kBits - ex : 0;
+        } else {
+            __syncthreads();
+        }
+        if (tid < nc) {
+            const int slot = rr_slot(cnt, NK, key, rank);
+            rs[slot] = s;
 #pragma unroll
-            for (int dy = 0; dy < (DIM > 1 ? NS : 1); ++dy) {
-                const T kyz = (DIM > 1 ? k2[dy] : T(1)) * kz;
-                const T cr = sr * kyz, ci = si * kyz;
-                const int row = ((l3 + dz) * tl.tw[1] + (l2 + dy)) * tl.tw[0] + l1;
+            for (int d = 0; d < DIM; ++d) rx[d * BT + slot] = x1[d];
+            rcr[slot] = cv.x;
+            rci[slot] = cv.y;
+        }
+        __syncthreads();
+        if (tid < nc) {
+            const int s0 = rs[tid];
+            T k1[NS], k2[NS], k3[NS];
+            taps_from_offset<T, NS>(rx[tid], kp, k1);
+            if (DIM > 1) taps_from_offset<T, NS>(rx[BT + tid], kp, k2);
+            if (DIM > 2) taps_from_offset<T, NS>(rx[2 * BT + tid], kp, k3);
+            const T sr = ldexp(rcr[tid], sh), si = ldexp(rci[tid], sh);
 #pragma unroll
-                for (int a = 0; a < NS; ++a) {
-                    FxAcc<Acc>::add(&sre[row + a], FxAcc<Acc>::cvt(cr * k1[a]));
-                    FxAcc<Acc>::add(&sim[row + a], FxAcc<Acc>::cvt(ci * k1[a]));
+            for (int dz = 0; dz < (DIM > 2 ? NS : 1); ++dz) {
+                const T kz = DIM > 2 ? k3[dz] : T(1);
+#pragma unroll
+                for (int dy = 0; dy < (DIM > 1 ? NS : 1); ++dy) {
+                    const T kyz = (DIM > 1 ? k2[dy] : T(1)) * kz;
+                    const T cr = sr * kyz, ci = si * kyz;
+                    const int row = s0 + (dz * tl.tw[1] + dy) * tl.tw[0];
+#pragma unroll
+                    for (int a = 0; a < NS; ++a) {
+                        FxAcc<Acc>::add(&sre[row + a], FxAcc<Acc>::cvt(cr * k1[a]));
+                        FxAcc<Acc>::add(&sim[row + a], FxAcc<Acc>::cvt(ci * k1[a]));
+                    }
                 }
+            }
+        }
+        if (DRAIN && ch + 1 < nch) {
+            __syncthreads();
+            const double back = ldexp(eb, -sh);
+            for (int p = tid; p < n; p += BT) {
+                const Acc ir = sre[p], ii = sim[p];
+                dre[p] += (float)((double)ir * back);
+                dimg[p] += (float)((double)ii * back);
+                sre[p] = 0;
+                sim[p] = 0;
             }
         }
     }
     __syncthreads();
-    // e^(DIM*beta) / S in FP64: in FP32 the rounding of DIM*beta (~50) alone
-    // would scale the whole tile by up to 2e-6, and 2^-sh can leave FP32 range
-    const double back = ldexp(exp((double)DIM * (double)kp.beta), -sh);
+    const double back = ldexp(eb, -sh);
     cpx<T> *f = fw + (long long)J.t * nftot;
-    for_tile_cells(tl, g, threadIdx.x, BT, [&](int q, long long off) {
-        const Acc ir = sre[q], ii = sim[q];
-        if (ir != 0 || ii != 0) {
-            cpx<T> *p = f + off;
-            gadd(&p->x, (T)((double)ir * back));
-            gadd(&p->y, (T)((double)ii * back));
+    const bool drained = DRAIN && nch > 1;
+    for_tile_cells(tl, g, tid, BT, [&](int p, long long off) {
+        double vr = (double)sre[p] * back, vi = (double)sim[p] * back;
+        if (drained) {
+            vr += (double)dre[p];
+            vi += (double)dimg[p];
+        }
+        if (vr != 0.0 || vi != 0.0) {
+            cpx<T> *q = f + off;
+            gadd(&q->x, (T)vr);
+            gadd(&q->y, (T)vi);
         }
     });
     __syncthreads();
 }
 
-// BT threads per workgroup: 256, or 1024 for large tiles with subproblems
-// of >= 1024 points (a tile over half the LDS leaves one workgroup per CU,
-// and the wider workgroup keeps the CU's wave count up for the zeroing, the
-// taps and the flush).
-template <typename T, int DIM, int NS, int BT>
+// SM spread kernel: persistent workgroups of BT threads take jobs from the
+// per-XCD queues.  Acc = int: 32-bit cells in chunks of BT points (BT within
+// the plan's 32-bit bound); Acc = long long: 64-bit cells.
+template <typename T, typename Acc, int DIM, int NS, int BT>
 __global__ void __launch_bounds__(BT)
 k_spread_sm(int M, SortedPts sp, const cpx<T> *__restrict__ c, cpx<T> *__restrict__ fw,
             BinGeom g, KerParams<T> kp, SubprobArgs sa, int ntr, long long nftot,
@@ -1072,13 +1048,8 @@ k_spread_sm(int M, SortedPts sp, const cpx<T> *__restrict__ c, cpx<T> *__restric
     jobq_run(
         jq, (long long)total * ntr,
         [&](long long job) {
-            const SubprobJob J = subprob_job(job, total, sa);
-            if (sizeof(T) == 4 && J.np <= kp.fx32max)
-                spread_sm_job_fx<T, int, DIM, NS, BT>(J, M, sp, c, fw, g, kp, nftot, smem,
-                                                      sa.maxsub);
-            else
-                spread_sm_job_fx<T, long long, DIM, NS, BT>(J, M, sp, c, fw, g, kp, nftot, smem,
-                                                            sa.maxsub);
+            spread_sm_job<T, Acc, DIM, NS, BT>(subprob_job(job, total, sa), M, sp, c, fw, g, kp,
+                                               nftot, smem);
         },
         [&](long long job) { return subprob_pos(job, total, M, sa); }, (long long)M * ntr);
 }
@@ -1126,8 +1097,8 @@ __device__ __forceinline__ int group_member(const long long *off, int n, long lo
 }
 
 // SM spread of a whole plan group in one persistent launch over the global
-// job queue of all members.
-template <typename T, int DIM, int NS>
+// job queue of all members (256-thread workgroups).
+template <typename T, typename Acc, int DIM, int NS>
 __global__ void __launch_bounds__(256)
 k_spread_sm_group(const GroupItem<T> *__restrict__ items, int n,
                   const long long *__restrict__ off, KerParams<T> kp, unsigned long long *jq)
@@ -1137,12 +1108,8 @@ k_spread_sm_group(const GroupItem<T> *__restrict__ items, int n,
         const int p = group_member(off, n, job);
         const GroupItem<T> &it = items[p];
         const SubprobJob J = subprob_job(job - off[p], it.sa.counters[1], it.sa);
-        if (sizeof(T) == 4 && J.np <= kp.fx32max)
-            spread_sm_job_fx<T, int, DIM, NS, 256>(J, it.M, it.sp, it.in, it.out, it.g, kp,
-                                                   it.nftot, smem, it.sa.maxsub);
-        else
-            spread_sm_job_fx<T, long long, DIM, NS, 256>(J, it.M, it.sp, it.in, it.out, it.g, kp,
-                                                         it.nftot, smem, it.sa.maxsub);
+        spread_sm_job<T, Acc, DIM, NS, 256>(J, it.M, it.sp, it.in, it.out, it.g, kp, it.nftot,
+                                            smem);
     });
 }
 
@@ -1164,13 +1131,13 @@ k_spread_sm_group(const GroupItem<T> *__restrict__ items, int n,
 // serialise (measured: 15x the address conflicts of arrival order).
 // ===========================================================================
 
-__device__ __forceinline__ int floor_div(int a, int b)
+CFA_HD int floor_div(int a, int b)
 {
     return a >= 0 ? a / b : -((-a + b - 1) / b);
 }
 
 // Along one axis: first tap and the unwrapped block frames kb0..kb1 touched.
-__device__ __forceinline__ void blk_frames(double v, int ns, int ob, int &start, int &kb0,
+CFA_HD void blk_frames(double v, int ns, int ob, int &start, int &kb0,
                                            int &kb1)
 {
     start = (int)ceil(v - 0.5 * ns);
@@ -1540,13 +1507,13 @@ k_interp_sm_group(const GroupItem<T> *__restrict__ items, int n,
 
 // Frequency of grid index l along an axis with ms modes and nf cells, or
 // INT_MIN if l holds no mode (inverse of the w(k) map of deconvolve_wrapper.cu:32-35).
-__device__ __forceinline__ int grid2freq(int l, int ms, int nf)
+CFA_HD int grid2freq(int l, int ms, int nf)
 {
     if (l <= (ms - 1) / 2) return l;
     if (l >= nf - ms / 2) return l - nf;
     return INT32_MIN;
 }
-__device__ __forceinline__ int freq2grid(int f, int nf) { return f >= 0 ? f : nf + f; }
+CFA_HD int freq2grid(int f, int nf) { return f >= 0 ? f : nf + f; }
 
 // fk[t][k] = fw[t][w(k)] / prod_d fwkerhalf_d[|k_d - ms_d/2|]
 // (Deconvolve_{2,3}d, deconvolve_wrapper.cu:26-66).  One workgroup per output
@@ -1582,12 +1549,15 @@ k_deconvolve(int ms, int mt, int mu, int n1, int n2, int n3, const cpx<T> *__res
 
 // Type-2 step 1 as one pass over the fine grid: every cell is written once,
 // with fk[k]/prod fwkerhalf at mode cells and 0 elsewhere (Amplify_{2,3}d plus
-// the memset before it, deconvolve_wrapper.cu:81-121,181-182).
+// the memset before it, deconvolve_wrapper.cu:81-121,181-182).  With wax >= 0
+// the modes are also multiplied by i * wsign * k_wax (k_wax the frequency
+// along axis wax): the type-2 transform of those modes is the derivative of
+// c_j = sum_k f_k e^{i wsign k.x_j} with respect to x_j along that axis.
 template <typename T, int DIM>
 __global__ void __launch_bounds__(256)
 k_amplify(int ms, int mt, int mu, int n1, int n2, int n3, cpx<T> *__restrict__ fw,
           const cpx<T> *__restrict__ fk, const T *__restrict__ ker1, const T *__restrict__ ker2,
-          const T *__restrict__ ker3, long long rows, long long ntot)
+          const T *__restrict__ ker3, long long rows, long long ntot, int wax, int wsign)
 {
     for (long long r = blockIdx.x; r < rows; r += gridDim.x) {
         const int l2 = DIM > 1 ? (int)(r % n2) : 0;
@@ -1606,6 +1576,8 @@ k_amplify(int ms, int mt, int mu, int n1, int n2, int n3, cpx<T> *__restrict__ f
         const T kyz = (DIM > 1 ? ker2[abs(f2)] : (T)1) * (DIM > 2 ? ker3[abs(f3)] : (T)1);
         const cpx<T> *src =
             fk + t * ntot + ((long long)(f3 + mu / 2) * mt + (f2 + mt / 2)) * (long long)ms;
+        // weight i * wsign * k of a fixed row axis (y or z), or per cell (x)
+        const T wrow = wax == 1 ? (T)(wsign * f2) : (wax == 2 ? (T)(wsign * f3) : (T)1);
         for (int l1 = threadIdx.x; l1 < n1; l1 += blockDim.x) {
             const int f1 = grid2freq(l1, ms, n1);
             cpx<T> o = zero;
@@ -1614,6 +1586,12 @@ k_amplify(int ms, int mt, int mu, int n1, int n2, int n3, cpx<T> *__restrict__ f
                 const cpx<T> v = src[f1 + ms / 2];
                 o.x = v.x / kv;
                 o.y = v.y / kv;
+                if (wax >= 0) {
+                    const T w = wax == 0 ? (T)(wsign * f1) : wrow;
+                    const T re = -o.y * w;
+                    o.y = o.x * w;
+                    o.x = re;
+                }
             }
             dst[l1] = o;
         }

@@ -25,19 +25,18 @@ struct SortBufs {
     int *binstart = nullptr;    // [nbins]
     int *substart = nullptr;    // [nbins + 1]
     int *s2b = nullptr;         // [nbins + ceil(M / maxsub)] upper bound
-    int *skey = nullptr;        // [M] sort keys (bin, or bin and cell)
-    int *sval = nullptr;        // [M] point indices fed to the sort
-    int *skey_out = nullptr;    // [M] sorted keys
-    void *sort_tmp = nullptr;   // radix sort scratch
-    size_t sort_tmp_bytes = 0;
+    int *skey = nullptr;        // [M] sort keys, pass ping-pong buffer 0
+    int *skey_out = nullptr;    // [M] sort keys, pass ping-pong buffer 1
+    void *rec[2] = {nullptr, nullptr};  // [M] point records between sort passes
+    int *sort_counts = nullptr; // (digit, tile) counts of one sort pass
+    void *sort_part = nullptr;  // int2 partial sums of their scan
     int *idx = nullptr;         // [M] caller index of each sorted point
     double *pos = nullptr;      // [dim][pos_stride] sorted rescaled positions
-    double *pack = nullptr;     // [M] input-order coordinates, one record per point (2D/3D)
     long long pos_stride = 0;   // capacity in points of each pos plane
     int *counters = nullptr;    // [2]: points, subproblems
     unsigned long long *jobq = nullptr;  // [kJobqWords] SM job queue words, zero between launches
     void *scan_part = nullptr;  // int2[ceil(nbins / chunk)]
-    // fine sort (key = bin * cells-per-bin + cell in bin), when enabled
+    // method 4: point copies counted per block
     int *kcount = nullptr;       // [nkeys]
     int *kstart = nullptr;       // [nkeys]
     void *kscan_part = nullptr;  // int2[ceil(nkeys / chunk)]
@@ -48,17 +47,50 @@ struct Launch {
     int ncu = 256;
 };
 
-// Bin sort of M points: stable radix sort of the bin keys (fine: bin and
-// fine-grid cell inside the bin), per-bin counts, scans, positions gathered
-// into sorted order, subproblem map.
+// Bin sort of M points: LSD counting sort of the bin keys (fine: bin and
+// fine-grid cell inside the bin) carrying the points, per-bin counts, scans,
+// sorted index and positions, subproblem map.
 template <typename T>
-int run_radix_sort(int dim, const BinGeom &g, long long nbins, int M, const T *x, const T *y,
-                   const T *z, int maxsub, bool fine, bool need_subprob, SortBufs &sb,
-                   const Launch &L);
-// Scratch bytes of that sort for M points and nkeys distinct keys.
-size_t radix_tmp_bytes(int M, long long nkeys);
-// Bytes per point of sb.pack for that sort (elem = sizeof(T); 0 in 1D).
-size_t pack_rec_bytes(int dim, int elem);
+int run_bin_sort(int dim, const BinGeom &g, long long nbins, int M, const T *x, const T *y,
+                 const T *z, int maxsub, bool fine, bool need_subprob, SortBufs &sb,
+                 const Launch &L);
+
+// Sort tiles: one workgroup of kSortWaves waves, kSortRounds rounds of 64
+// points per wave; digits of at most kMaxDigitBits bits.
+constexpr int kSortWaves = 4;
+constexpr int kSortRounds = 16;
+constexpr int kSortTile = 4096;
+constexpr int kMaxDigitBits = 10;
+
+// Pass structure of that sort for M points and nkeys distinct keys: key
+// bits, digit bits per pass, passes, tiles, (digit, tile) counts per pass.
+// The fewest passes of at most kMaxDigitBits bits, with one pass's count
+// array within 2^27 entries.
+struct BinSortShape {
+    int bits = 1, digit_bits = 1, passes = 1, tiles = 0;
+    long long counts = 0;
+};
+inline BinSortShape bin_sort_shape(int M, long long nkeys)
+{
+    BinSortShape s;
+    int bits = 1;
+    while ((1ll << bits) < nkeys) ++bits;
+    s.bits = bits;
+    s.tiles = (int)(((long long)M + kSortTile - 1) / kSortTile);
+    int dmax = kMaxDigitBits;
+    while (dmax > 4 && ((long long)s.tiles << dmax) > (1ll << 27)) --dmax;
+    s.passes = (bits + dmax - 1) / dmax;
+    s.digit_bits = (bits + s.passes - 1) / s.passes;
+    s.counts = (long long)s.tiles << s.digit_bits;
+    return s;
+}
+// Bytes of one point record moved between passes: dim coordinates of elem
+// bytes and an int index, aligned to elem.
+inline size_t bin_sort_rec_bytes(int dim, int elem)
+{
+    const size_t raw = (size_t)dim * elem + 4;
+    return (raw + elem - 1) / elem * elem;
+}
 // Natural order (gpu_sort = 0): identity index and positions.
 template <typename T>
 int run_natural_order(int dim, const BinGeom &g, int M, const T *x, const T *y, const T *z,
@@ -84,33 +116,88 @@ int run_interp(int method, int dim, int ns, int M, const SortBufs &sb, cpx<T> *c
                const cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp, int maxsub,
                long long sub_bound, int ntr, long long nftot, const Launch &L);
 
-// Bytes of LDS the SM kernels need for a tile (re and im of every cell).
-size_t sm_tile_bytes(int dim, int ns, const int *bs, size_t elem);
-// LDS bytes of the SM spread: 32- or 64-bit fixed-point tile plus staged
-// strengths for single precision (elem 4), FP64 tile for double precision.
-size_t spread_lds_bytes(int dim, int ns, const int *bs, size_t elem, int maxsub, int fxwide);
+// Cells of the SM tile of a bin: the bin plus a halo of ceil(ns/2) per side.
+inline size_t sm_tile_cells(int dim, int ns, const int *bs)
+{
+    const int P = (ns + 1) / 2;
+    size_t cells = (size_t)(bs[0] + 2 * P);
+    if (dim > 1) cells *= (size_t)(bs[1] + 2 * P);
+    if (dim > 2) cells *= (size_t)(bs[2] + 2 * P);
+    return cells;
+}
+// Bytes of LDS the SM interp needs for a tile (a complex T per cell).
+inline size_t sm_tile_bytes(int dim, int ns, const int *bs, size_t elem)
+{
+    return 2 * elem * sm_tile_cells(dim, ns, bs);
+}
+
+// LDS bytes of one SM spread job with bt threads: re / im accumulator planes
+// (acc_bytes per cell each), the staged chunk of bt points (start cell, dim
+// first-tap offsets and the complex strength, elem = sizeof(T)), 64 bank
+// counters, one reduction value per wave, and last the FP32 drain planes,
+// which only jobs of more than bt points use (32-bit cells with subproblems
+// larger than one chunk).
+inline size_t spread_job_lds(size_t cells, int acc_bytes, int dim, int elem, int bt, bool drain)
+{
+    return cells * 2 * acc_bytes + (size_t)bt * (4 + (size_t)(dim + 2) * elem) + 64 * 4 +
+           (size_t)(bt / 64) * elem + (drain ? cells * 8 : 0);
+}
+// Largest number of points one 32-bit fixed-point spread scale may cover at
+// tolerance tol: 4 n 2^-30 <= tol / 5 (536 at tol 1e-5).
+inline int fx32_bound(double tol)
+{
+    double n = tol / 5.0 / (4.0 / 1073741824.0);
+    n = n < 0 ? 0 : (n > 2e9 ? 2e9 : n);
+    return (int)n;
+}
+
+// Launch shape of the SM spread: threads per workgroup, accumulator bytes
+// per cell (4: 32-bit fixed point drained between chunks, 8: 64-bit) and
+// dynamic LDS bytes, for a plan (group = false) or a plan group.
+struct SpreadCfg {
+    int bt = 256;
+    int acc_bytes = 8;
+    size_t lds = 0;
+};
+inline SpreadCfg spread_cfg(int dim, int ns, const int *bs, size_t elem, int fx32max,
+                            int maxsub, bool group)
+{
+    const size_t cells = sm_tile_cells(dim, ns, bs);
+    SpreadCfg cfg;
+    // 32-bit cells when a whole workgroup-wide chunk stays within the plan's
+    // 32-bit bound (BT = 512, or 256 in plan groups, whose kernels run 256
+    // threads); 64-bit cells otherwise
+    const int chunk = group ? 256 : 512;
+    if (elem == 4 && fx32max >= chunk) {
+        cfg.acc_bytes = 4;
+        cfg.bt = chunk;
+    } else {
+        cfg.acc_bytes = 8;
+        // 1024 threads when the tile leaves one workgroup per CU at any
+        // width (over half the LDS) and subproblems fill it: the wider
+        // workgroup adds waves without taking concurrent jobs away
+        const size_t l256 = spread_job_lds(cells, 8, dim, (int)elem, 256, false);
+        const size_t l1024 = spread_job_lds(cells, 8, dim, (int)elem, 1024, false);
+        cfg.bt = !group && maxsub >= 1024 && l256 > kMaxLdsBytes / 2 && l1024 <= kMaxLdsBytes
+                     ? 1024
+                     : 256;
+    }
+    cfg.lds = spread_job_lds(cells, cfg.acc_bytes, dim, (int)elem, cfg.bt,
+                             cfg.acc_bytes == 4 && maxsub > cfg.bt);
+    return cfg;
+}
 // LDS bytes of a method-4 (owner-computes block) spread job over blocks ob.
 size_t blk_lds_bytes(int dim, const int *ob, size_t elem, int maxsub, int fxwide);
 
 template <typename T>
 int run_deconvolve(int dim, const int *ms, const int *nf, const cpx<T> *fw, cpx<T> *fk,
                    const T *const *ker, int ntr, const Launch &L);
+// wax >= 0 also multiplies the modes by i * wsign * k_wax (derivative modes).
 template <typename T>
 int run_amplify(int dim, const int *ms, const int *nf, cpx<T> *fw, const cpx<T> *fk,
-                const T *const *ker, int ntr, const Launch &L);
+                const T *const *ker, int ntr, const Launch &L, int wax = -1, int wsign = 1);
 template <typename T>
 int run_zero(cpx<T> *p, long long n, const Launch &L);
-
-// Interp of the value and its DIM coordinate derivatives (differentiable
-// form): dc holds DIM planes of dc_plane complex values.
-template <typename T>
-int run_interp_grad(int dim, int ns, int M, const SortBufs &sb, cpx<T> *c, cpx<T> *dc,
-                    const cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp, int ntr,
-                    long long nftot, long long dc_plane, const Launch &L);
-template <typename T, int DIM>
-int interp_grad_dim(int ns, int M, const SortBufs &sb, cpx<T> *c, cpx<T> *dc, const cpx<T> *fw,
-                    const BinGeom &g, const KerParams<T> &kp, int ntr, long long nftot,
-                    long long dc_plane, const Launch &L);
 
 // Plan groups: one persistent spread (type 1) or interp (type 2) launch over
 // the subproblems of every member.  items is a device array of n

@@ -97,27 +97,35 @@ int setup_spreader(Spread<T> &sp, T eps, T upsampfac, int kerevalmeth)
     return ier;
 }
 
-// Smallest even 2,3,5-smooth integer >= n (contrib/utils.cpp:3-22).
-int next235even(int n)
+static bool smooth235(long long m)
 {
-    if (n <= 2) return 2;
-    if (n % 2) ++n;
-    for (int m = n;; m += 2) {
-        int r = m;
-        while (r % 2 == 0) r /= 2;
-        while (r % 3 == 0) r /= 3;
-        while (r % 5 == 0) r /= 5;
-        if (r == 1) return m;
-    }
+    if (m < 1) return false;
+    while (m % 2 == 0) m /= 2;
+    while (m % 3 == 0) m /= 3;
+    while (m % 5 == 0) m /= 5;
+    return m == 1;
 }
 
-// Fine-grid size of one axis (SET_NF_TYPE12, contrib/common.cpp:24-37).
-int fine_size(int ms, double upsampfac, int ns)
+// Smallest even 2,3,5-smooth integer >= n that is a multiple of b
+// (next235beven, contrib/utils.cpp:3-22; n <= 2 gives 2 as there).  b must
+// be 2,3,5-smooth itself (the reference's search never ends otherwise), so
+// the candidates are the even multiples of b from n up.
+long long next235beven(long long n, long long b)
+{
+    if (n <= 2) return 2;
+    const long long step = b % 2 == 0 ? b : 2 * b;
+    for (long long m = (n + step - 1) / step * step;; m += step)
+        if (smooth235(m)) return m;
+}
+
+// Fine-grid size of one axis (SET_NF_TYPE12, contrib/common.cpp:24-37): b is
+// the output block size for method 4, else 1.
+int fine_size(int ms, double upsampfac, int ns, int b)
 {
     long long nf = (long long)(upsampfac * ms);
     if (nf < 2 * ns) nf = 2 * ns;
-    if (nf < 2147483647LL) nf = next235even((int)nf);
-    return (int)nf;
+    if (nf < 2147483647LL) nf = next235beven(nf, b);
+    return nf > 2147483647LL ? 2147483647 : (int)nf;
 }
 
 // ---------------------------------------------------------------------------
@@ -199,20 +207,17 @@ static void fseries(int nf, const Spread<T> &sp, std::vector<double> &out)
     }
 }
 
-// Accumulator width of the FP32 fixed-point spread tile.  A 32-bit tile
-// rounds every tap to A * 2^-31 with A the strength mass of the job, so its
-// error grows with the points per subproblem: CPU emulation of the tile on a
-// clustered 3D tol-1e-6 case adds about 2-3 * maxsub * 2^-30 to the relative
-// error of the transform.  The 32-bit tile is kept while 4 * maxsub * 2^-30
-// stays within tol/5; otherwise the tile is 64-bit (ds_add_u64).  The rule
-// applies per job: fx32_max is the largest point count that keeps 32 bits,
-// and fx_wide says whether the plan's largest jobs exceed it (which sizes
-// the LDS tile for 64-bit cells).
-static int fx32_max(double tol)
-{
-    const double n = std::floor(tol / 5.0 / (4.0 * std::ldexp(1.0, -30)));
-    return n > 2e9 ? 2000000000 : (int)n;
-}
+// Accumulator width of the FP32 fixed-point spread tiles.  A 32-bit tile
+// rounds every tap to A * 2^-31 with A the strength mass of the points
+// accumulated under one scale, so its error grows with their number n: CPU
+// emulation of the tile on a clustered 3D tol-1e-6 case adds about
+// 2-3 * n * 2^-30 to the relative error of the transform.  fx32_max is the
+// largest n with 4 * n * 2^-30 within tol/5 (536 at tol 1e-5).  The SM
+// spread accumulates 32-bit chunks of at most that many points, each under
+// its own scale and drained to FP32 between chunks, when a workgroup-wide
+// chunk fits the bound, else 64-bit cells.  Method 4 takes 32 bits for the
+// whole job when its subproblems fit the bound (fx_wide = 0).
+static int fx32_max(double tol) { return fx32_bound(tol); }
 static int fx_wide(int maxsub, double tol) { return maxsub > fx32_max(tol) ? 1 : 0; }
 
 // ---------------------------------------------------------------------------
@@ -369,8 +374,8 @@ struct Plan {
     T *hcoef = nullptr;
     SortBufs sb;
 
-    int fxwide = 0;   // FP32 SM spread tile sized for 64-bit (1) or 32-bit (0) cells
-    int fx32max = 0;  // FP32: jobs of at most this many points use the 32-bit tile
+    int fxwide = 0;   // FP32 method 4: 64-bit (1) or 32-bit (0) block tiles
+    int fx32max = 0;  // FP32: most points one 32-bit fixed-point scale may cover
 
     KerParams<T> ker_params() const
     {
@@ -440,13 +445,16 @@ struct Plan {
         dfree(sb.substart);
         dfree(sb.s2b);
         dfree(sb.skey);
-        dfree(sb.sval);
         dfree(sb.skey_out);
-        if (sb.sort_tmp) hipFree(sb.sort_tmp);
-        sb.sort_tmp = nullptr;
+        for (auto &r : sb.rec) {
+            if (r) hipFree(r);
+            r = nullptr;
+        }
+        dfree(sb.sort_counts);
+        if (sb.sort_part) hipFree(sb.sort_part);
+        sb.sort_part = nullptr;
         dfree(sb.idx);
         dfree(sb.pos);
-        dfree(sb.pack);
         dfree(sb.counters);
         dfree(sb.jobq);
         if (sb.scan_part) hipFree(sb.scan_part);
@@ -491,10 +499,12 @@ struct Plan {
     }
 
     int makeplan(int type_, int dim_, const int *nmodes, int iflag_, int ntransf_, T tol,
-                 int maxbatch_, const cufinufft_opts *o);
+                 int maxbatch_, const cufinufft_opts *o, int flags = 0);
     int setpts(int M_, const T *x, const T *y, const T *z);
     int setpts_blocks();
     int cap_copies = -1;  // capacity of idx / pos in point copies (method 4)
+    long long cap_counts = -1;          // capacity of sb.sort_counts
+    long long cap_rec[2] = {-1, -1};    // capacity of sb.rec[] in points
     int execute(cpx<T> *c, cpx<T> *fk);
     int execute_grad(cpx<T> *c, cpx<T> *dc, cpx<T> *fk);
 };
@@ -556,10 +566,18 @@ static void default_binsizes(int dim, cufinufft_opts &o)
 
 template <typename T>
 int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int ntransf_, T tol,
-                      int maxbatch_, const cufinufft_opts *o)
+                      int maxbatch_, const cufinufft_opts *o, int flags)
 {
     if (type_ != 1 && type_ != 2) return 1;
     if (dim_ < 1 || dim_ > 3) return 1;
+    if (flags & ~CUFINUFFT_ACCURACY_FIRST) return 1;
+    // Accuracy-first: the reference's width rule ns = ceil(-log10(tol/10))
+    // leaves the transform's relative l2 error at 0.6-1.1, 1-2 and 1.5-3.2
+    // times tol in 1, 2 and 3 dimensions (the same algorithm in FP64 against
+    // direct sums, tol 1e-5 and 1e-9).  Planning for tol * 2 / (dim + 2)
+    // instead (tol/2.5 in 3D: one more tap per axis at tol 1e-5 and 1e-9)
+    // brings it to 0.1-0.4 times tol.
+    if (flags & CUFINUFFT_ACCURACY_FIRST) tol = (T)((double)tol * 2.0 / (dim_ + 2));
     if (!nmodes) return kErrNdataNotValid;
     if (o) {
         opts = *o;
@@ -584,6 +602,8 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
     eps_warning = ier == kWarnEpsTooSmall;
     if (opts.gpu_kerevalmeth != 0 && opts.gpu_kerevalmeth != 1) return 1;
 
+    const bool bin_default[3] = {opts.gpu_binsizex < 0, opts.gpu_binsizey < 0,
+                                 opts.gpu_binsizez < 0};
     default_binsizes(dim, opts);
     if (opts.gpu_binsizex < 1 || opts.gpu_binsizey < 1 || opts.gpu_binsizez < 1) return 1;
     iflag = iflag_ >= 0 ? 1 : -1;
@@ -592,12 +612,16 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
     if (maxbatch < 1) return kErrNdataNotValid;
     maxbatch = std::min(maxbatch, ntransf);
 
+    const int obv[3] = {opts.gpu_obinsizex, opts.gpu_obinsizey, opts.gpu_obinsizez};
+    if (opts.gpu_method == 4)
+        for (int d = 0; d < dim; ++d)
+            if (!smooth235(obv[d])) return 1;  // nf must become a multiple of the block
     for (int d = 0; d < dim; ++d) {
         if (opts.gpu_spreadinterponly) {
             nf[d] = ms[d];
             if (nf[d] < 2 * sp.ns) return kErrSpreadBoxSmall;
         } else {
-            nf[d] = fine_size(ms[d], opts.upsampfac, sp.ns);
+            nf[d] = fine_size(ms[d], opts.upsampfac, sp.ns, opts.gpu_method == 4 ? obv[d] : 1);
         }
     }
     nftot = (long long)nf[0] * nf[1] * nf[2];
@@ -612,17 +636,39 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
     // plans); the interp tile holds T.
     fxwide = fx_wide(opts.gpu_maxsubprobsize, (double)tol);
     fx32max = sizeof(T) == 4 ? fx32_max((double)tol) : 0;
-    const int bsv[3] = {opts.gpu_binsizex, opts.gpu_binsizey, opts.gpu_binsizez};
-    const size_t tile_bytes =
-        type == 1 ? spread_lds_bytes(dim, sp.ns, bsv, sizeof(T), opts.gpu_maxsubprobsize, fxwide)
-                  : sm_tile_bytes(dim, sp.ns, bsv, sizeof(T));
+    int bsv[3] = {opts.gpu_binsizex, opts.gpu_binsizey, opts.gpu_binsizez};
+    auto tile_lds = [&]() {
+        return type == 1 ? spread_cfg(dim, sp.ns, bsv, sizeof(T), fx32max,
+                                      opts.gpu_maxsubprobsize, false)
+                               .lds
+                         : sm_tile_bytes(dim, sp.ns, bsv, sizeof(T));
+    };
+    size_t tile_bytes = tile_lds();
+    // Default bins (the reference's sizes) whose SM tile does not fit the LDS
+    // with this kernel width (e.g. FP64 3D type 1 at ns 11) are halved along
+    // their largest defaulted side until it fits; bins the caller chose are
+    // kept as given.
+    if ((opts.gpu_method == 2 || opts.gpu_method == 0) && tile_bytes > kMaxLdsBytes) {
+        while (tile_bytes > kMaxLdsBytes) {
+            int k = -1;
+            for (int d = 0; d < dim; ++d)
+                if (bin_default[d] && bsv[d] > 2 && (k < 0 || bsv[d] > bsv[k])) k = d;
+            if (k < 0) break;
+            bsv[k] /= 2;
+            tile_bytes = tile_lds();
+        }
+        opts.gpu_binsizex = bsv[0];
+        opts.gpu_binsizey = bsv[1];
+        opts.gpu_binsizez = bsv[2];
+    }
     method = opts.gpu_method;
     if (method == 0) method = tile_bytes <= kMaxLdsBytes ? 2 : 1;
     if (method == 2 && tile_bytes > kMaxLdsBytes) return kErrLds;
-    const int obv[3] = {opts.gpu_obinsizex, opts.gpu_obinsizey, opts.gpu_obinsizez};
     if (method == 4) {
-        // blocks must tile the fine grid and be whole numbers of bins, as
-        // in the reference (spread3d_wrapper.cu:410-441)
+        // blocks must tile the fine grid (nf was rounded to a multiple of
+        // the block above, except in spread/interp-only mode, where nf is the
+        // caller's grid) and be whole numbers of bins, as in the reference
+        // (spread3d_wrapper.cu:410-441)
         for (int d = 0; d < dim; ++d)
             if (obv[d] < 1 || nf[d] % obv[d] != 0 || obv[d] % bsv[d] != 0) return 1;
         if (blk_lds_bytes(dim, obv, sizeof(T), opts.gpu_maxsubprobsize, fxwide) > kMaxLdsBytes)
@@ -752,31 +798,44 @@ int Plan<T>::setpts(int M_, const T *x, const T *y, const T *z)
     kz = dim > 2 ? z : nullptr;
     if (M > cap_M && method != 4) {
         dfree(sb.skey);
-        dfree(sb.sval);
         dfree(sb.skey_out);
         dfree(sb.idx);
         dfree(sb.pos);
-        dfree(sb.pack);
-        const size_t pack_doubles = (size_t)M * pack_rec_bytes(dim, sizeof(T)) / 8;
-        if (dmalloc(&sb.skey, M) || dmalloc(&sb.sval, M) || dmalloc(&sb.skey_out, M) ||
-            dmalloc(&sb.idx, M) || dmalloc(&sb.pos, (size_t)M * dim) ||
-            (pack_doubles && dmalloc(&sb.pack, pack_doubles))) {
+        if (dmalloc(&sb.skey, M) || dmalloc(&sb.skey_out, M) || dmalloc(&sb.idx, M) ||
+            dmalloc(&sb.pos, (size_t)M * dim)) {
             cap_M = -1;
             return kErrHip;
         }
         cap_M = M;
         sb.pos_stride = M;
     }
-    if (method != 4 && !(method == 1 && !opts.gpu_sort)) {
+    if (method != 4 && !(method == 1 && !opts.gpu_sort) && M > 0) {
+        // counting-sort scratch: (digit, tile) counts and their scan
+        // partials, and the point records between passes (none for one pass)
         const long long nkeys =
             opts.gpu_sort == 2 ? nbins * (long long)geom.bs[0] * geom.bs[1] * geom.bs[2] : nbins;
-        const size_t need = radix_tmp_bytes(std::max(M, 1), nkeys);
-        if (need > sb.sort_tmp_bytes) {
-            if (sb.sort_tmp) hipFree(sb.sort_tmp);
-            sb.sort_tmp = nullptr;
-            sb.sort_tmp_bytes = 0;
-            if (hipMalloc(&sb.sort_tmp, need) != hipSuccess) return kErrHip;
-            sb.sort_tmp_bytes = need;
+        const BinSortShape sh = bin_sort_shape(M, nkeys);
+        if (sh.counts > cap_counts) {
+            dfree(sb.sort_counts);
+            if (sb.sort_part) hipFree(sb.sort_part);
+            sb.sort_part = nullptr;
+            cap_counts = -1;
+            const long long parts = (sh.counts + kScanChunk - 1) / kScanChunk;
+            if (dmalloc(&sb.sort_counts, (size_t)sh.counts) ||
+                hipMalloc(&sb.sort_part, sizeof(int2) * (size_t)parts) != hipSuccess)
+                return kErrHip;
+            cap_counts = sh.counts;
+        }
+        const size_t rb = bin_sort_rec_bytes(dim, sizeof(T));
+        for (int r = 0; r < 2; ++r) {
+            const long long need = sh.passes > r + 1 ? M : 0;  // pass r writes records r
+            if (need > cap_rec[r]) {
+                if (sb.rec[r]) hipFree(sb.rec[r]);
+                sb.rec[r] = nullptr;
+                cap_rec[r] = -1;
+                if (hipMalloc(&sb.rec[r], rb * (size_t)need) != hipSuccess) return kErrHip;
+                cap_rec[r] = need;
+            }
         }
     }
     const int maxsub = opts.gpu_maxsubprobsize;
@@ -796,8 +855,8 @@ int Plan<T>::setpts(int M_, const T *x, const T *y, const T *z)
     if (method == 1 && !opts.gpu_sort) {
         e = run_natural_order<T>(dim, geom, M, kx, ky, kz, sb, L);
     } else {
-        e = run_radix_sort<T>(dim, geom, nbins, M, kx, ky, kz, maxsub, opts.gpu_sort == 2,
-                              need_sub, sb, L);
+        e = run_bin_sort<T>(dim, geom, nbins, M, kx, ky, kz, maxsub, opts.gpu_sort == 2,
+                            need_sub, sb, L);
     }
     tend(kStageSort);
     if (e) return e;
@@ -878,38 +937,40 @@ int Plan<T>::execute(cpx<T> *c, cpx<T> *fk)
 
 // Type-2 execute that also returns the derivatives of every output with
 // respect to its point's coordinates: dc[d][t*M + j] = d c[t*M + j] / d x_d(j).
+// Each derivative is itself a type-2 transform, of the modes weighted by
+// i * iflag * k_d (the weight is applied in the amplify pass), so it carries
+// the transform's own error bound: amplify, FFT and the plan's regular
+// interpolation run once for the values and once per dimension.
 template <typename T>
 int Plan<T>::execute_grad(cpx<T> *c, cpx<T> *dc, cpx<T> *fk)
 {
-    if (type != 2 || !pts_set) return 1;
+    if (type != 2 || !pts_set || opts.gpu_spreadinterponly) return 1;
     if ((M > 0 && (!c || !dc)) || !fk) return 1;
     DeviceGuard dg(opts.gpu_device_id);
-    const KerParams<T> kp{sp.beta, sp.c, sp.hw, nullptr, 0};  // value + derivative: exp(sqrt()) form
+    const KerParams<T> kp = ker_params();
+    const int maxsub = opts.gpu_maxsubprobsize;
     const T *kers[3] = {fwker[0], fwker[1], fwker[2]};
     const long long plane = (long long)M * ntransf;
     int e = 0;
-    for (int b0 = 0; b0 < ntransf; b0 += maxbatch) {
-        const int blk = std::min(maxbatch, ntransf - b0);
-        cpx<T> *cb = c + (long long)b0 * M;
-        cpx<T> *db = dc + (long long)b0 * M;
-        cpx<T> *grid;
-        if (opts.gpu_spreadinterponly) {
-            grid = fk + (long long)b0 * nftot;
-        } else {
-            grid = fw;
+    for (int comp = 0; comp <= dim; ++comp) {  // 0: values, 1 + d: d/dx_d
+        cpx<T> *out = comp == 0 ? c : dc + (long long)(comp - 1) * plane;
+        for (int b0 = 0; b0 < ntransf; b0 += maxbatch) {
+            const int blk = std::min(maxbatch, ntransf - b0);
             tbegin();
-            e = run_amplify<T>(dim, ms, nf, fw, fk + (long long)b0 * ntot, kers, blk, L);
+            e = run_amplify<T>(dim, ms, nf, fw, fk + (long long)b0 * ntot, kers, blk, L, comp - 1,
+                               iflag);
             tend(kStageDeconv);
             if (e) return e;
             tbegin();
             e = fft(fw, blk);
             tend(kStageFft);
             if (e) return e;
+            tbegin();
+            e = run_interp<T>(method, dim, sp.ns, M, sb, out + (long long)b0 * M, fw, geom, kp,
+                              maxsub, sub_bound, blk, nftot, L);
+            tend(kStageSpreadInterp);
+            if (e) return e;
         }
-        tbegin();
-        e = run_interp_grad<T>(dim, sp.ns, M, sb, cb, db, grid, geom, kp, blk, nftot, plane, L);
-        tend(kStageSpreadInterp);
-        if (e) return e;
     }
     return 0;
 }
@@ -1054,10 +1115,9 @@ struct Group {
         int e = 0;
         const Plan<T> &p0 = *plans[0];
         KerParams<T> kp = p0.ker_params();
-        for (const auto *p : plans) {  // the widest member's tile, the strictest 32-bit bound
-            kp.fxwide |= p->fxwide;
-            kp.fx32max = std::min(kp.fx32max, p->fx32max);
-        }
+        // the strictest member's 32-bit bound applies to every job of the
+        // group (members may differ in tol and maxsub)
+        for (const auto *p : plans) kp.fx32max = std::min(kp.fx32max, p->fx32max);
         const int n = (int)plans.size();
         std::vector<hipStream_t> saved(n);
         for (int i = 0; i < n; ++i) {
@@ -1169,7 +1229,7 @@ namespace {
 
 template <typename P, typename T>
 int makeplan_impl(int type, int dim, int *nmodes, int iflag, int ntransf, T tol, int maxbatch,
-                  P **out, cufinufft_opts *opts)
+                  P **out, cufinufft_opts *opts, int flags = 0)
 {
     if (!out) return 1;
     *out = nullptr;
@@ -1177,7 +1237,7 @@ int makeplan_impl(int type, int dim, int *nmodes, int iflag, int ntransf, T tol,
     if (!p) return 1;
     // Any failure frees the plan; a successful plan may still carry the
     // reference's WARN_EPS_TOO_SMALL (tol clamped to machine precision).
-    const int ier = p->makeplan(type, dim, nmodes, iflag, ntransf, tol, maxbatch, opts);
+    const int ier = p->makeplan(type, dim, nmodes, iflag, ntransf, tol, maxbatch, opts, flags);
     if (ier != 0) {
         delete p;
         return ier;
@@ -1272,6 +1332,20 @@ int cufinufftf_makeplan(int type, int dim, int *nmodes, int iflag, int ntransf, 
 {
     return makeplan_impl<cufinufftf_plan_s, float>(type, dim, nmodes, iflag, ntransf, tol,
                                                    maxbatchsize, plan, opts);
+}
+
+int cufinufft_makeplan_ext(int type, int dim, int *nmodes, int iflag, int ntransf, double tol,
+                           int maxbatchsize, cufinufft_plan *plan, cufinufft_opts *opts, int flags)
+{
+    return makeplan_impl<cufinufft_plan_s, double>(type, dim, nmodes, iflag, ntransf, tol,
+                                                   maxbatchsize, plan, opts, flags);
+}
+int cufinufftf_makeplan_ext(int type, int dim, int *nmodes, int iflag, int ntransf, float tol,
+                            int maxbatchsize, cufinufftf_plan *plan, cufinufft_opts *opts,
+                            int flags)
+{
+    return makeplan_impl<cufinufftf_plan_s, float>(type, dim, nmodes, iflag, ntransf, tol,
+                                                   maxbatchsize, plan, opts, flags);
 }
 
 int cufinufft_setpts(int M, double *x, double *y, double *z, int, double *, double *, double *,

@@ -71,28 +71,32 @@ int spread_ns(int method, int M, const SortBufs &sb, const cpx<T> *c, cpx<T> *fw
         k_spread_blk<T, DIM, NS><<<grid, 256, lds, L.stream>>>(M, sorted_pts(sb), c, fw, g, kp,
                                                                sa, ntr, nftot);
     } else {
-        const size_t lds = spread_lds_bytes(DIM, NS, g.bs, sizeof(T), maxsub, kp.fxwide);
-        if (lds > kMaxLdsBytes) return kErrLds;
+        const SpreadCfg cfg = spread_cfg(DIM, NS, g.bs, sizeof(T), kp.fx32max, maxsub, false);
+        if (cfg.lds > kMaxLdsBytes) return kErrLds;
         int grid = 0;
         SubprobArgs sa{sb.s2b, sb.substart, sb.binstart, sb.binsize, sb.counters, maxsub};
-        // 1024-thread workgroups when the tile admits only one workgroup per
-        // CU anyway (over half the LDS), so the wider workgroup adds waves
-        // without taking concurrent jobs away, and subproblems fill it
-        // (measured on 1e8 3D FP32 points: 16x16x8 tiles with 64-bit cells,
-        // 26.5 -> 22.9 ms; 16x16x2 tiles, two per CU at 256 threads but one at
-        // 1024 (the kernel's registers allow 28 waves per CU), 39.7 -> 44.1 ms)
-        if (maxsub >= 1024 && lds > kMaxLdsBytes / 2) {
-            int e = sm_grid<k_spread_sm<T, DIM, NS, 1024>>(lds, sub_bound * ntr, L.ncu, &grid,
-                                                           1024);
-            if (e) return e;
-            k_spread_sm<T, DIM, NS, 1024><<<grid, 1024, lds, L.stream>>>(
-                M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);
+#define CFA_SM_SPREAD(ACC, BT)                                                                \
+    do {                                                                                      \
+        int e = sm_grid<k_spread_sm<T, ACC, DIM, NS, BT>>(cfg.lds, sub_bound * ntr, L.ncu,    \
+                                                           &grid, BT);                        \
+        if (e) return e;                                                                      \
+        k_spread_sm<T, ACC, DIM, NS, BT><<<grid, BT, cfg.lds, L.stream>>>(                    \
+            M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);                        \
+    } while (0)
+        if constexpr (sizeof(T) == 4) {
+            if (cfg.acc_bytes == 4)
+                CFA_SM_SPREAD(int, 512);
+            else if (cfg.bt == 1024)
+                CFA_SM_SPREAD(long long, 1024);
+            else
+                CFA_SM_SPREAD(long long, 256);
         } else {
-            int e = sm_grid<k_spread_sm<T, DIM, NS, 256>>(lds, sub_bound * ntr, L.ncu, &grid);
-            if (e) return e;
-            k_spread_sm<T, DIM, NS, 256><<<grid, 256, lds, L.stream>>>(
-                M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);
+            if (cfg.bt == 1024)
+                CFA_SM_SPREAD(long long, 1024);
+            else
+                CFA_SM_SPREAD(long long, 256);
         }
+#undef CFA_SM_SPREAD
     }
     return hipGetLastError() == hipSuccess ? 0 : kErrHip;
 }
@@ -140,12 +144,21 @@ int group_ns(int type, const GroupItem<T> *items, int n, const long long *off,
     unsigned long long *jq =
         reinterpret_cast<unsigned long long *>(const_cast<long long *>(off) + n + 1);
     if (type == 1) {
-        const size_t lds = spread_lds_bytes(DIM, NS, bs, sizeof(T), maxsub, kp.fxwide);
-        if (lds > kMaxLdsBytes) return kErrLds;
+        const SpreadCfg cfg = spread_cfg(DIM, NS, bs, sizeof(T), kp.fx32max, maxsub, true);
+        if (cfg.lds > kMaxLdsBytes) return kErrLds;
         int grid = 0;
-        int e = sm_grid<k_spread_sm_group<T, DIM, NS>>(lds, jobs_bound, L.ncu, &grid);
-        if (e) return e;
-        k_spread_sm_group<T, DIM, NS><<<grid, 256, lds, L.stream>>>(items, n, off, kp, jq);
+        if (sizeof(T) == 4 && cfg.acc_bytes == 4) {
+            int e = sm_grid<k_spread_sm_group<T, int, DIM, NS>>(cfg.lds, jobs_bound, L.ncu, &grid);
+            if (e) return e;
+            k_spread_sm_group<T, int, DIM, NS><<<grid, 256, cfg.lds, L.stream>>>(items, n, off, kp,
+                                                                                jq);
+        } else {
+            int e = sm_grid<k_spread_sm_group<T, long long, DIM, NS>>(cfg.lds, jobs_bound, L.ncu,
+                                                                      &grid);
+            if (e) return e;
+            k_spread_sm_group<T, long long, DIM, NS><<<grid, 256, cfg.lds, L.stream>>>(
+                items, n, off, kp, jq);
+        }
     } else {
         const size_t lds = sm_tile_bytes(DIM, NS, bs, sizeof(T));
         if (lds > kMaxLdsBytes) return kErrLds;
@@ -154,16 +167,6 @@ int group_ns(int type, const GroupItem<T> *items, int n, const long long *off,
         if (e) return e;
         k_interp_sm_group<T, DIM, NS><<<grid, 256, lds, L.stream>>>(items, n, off, kp, jq);
     }
-    return hipGetLastError() == hipSuccess ? 0 : kErrHip;
-}
-
-template <typename T, int DIM, int NS>
-int interp_grad_ns(int M, const SortBufs &sb, cpx<T> *c, cpx<T> *dc, const cpx<T> *fw,
-                   const BinGeom &g, const KerParams<T> &kp, int ntr, long long nftot,
-                   long long dc_plane, const Launch &L)
-{
-    k_interp_grad<T, DIM, NS><<<grid_for_points(M, L.ncu), 256, 0, L.stream>>>(
-        M, sorted_pts(sb), c, dc, fw, g, kp, ntr, nftot, dc_plane);
     return hipGetLastError() == hipSuccess ? 0 : kErrHip;
 }
 
@@ -205,21 +208,6 @@ int interp_dim(int method, int ns, int M, const SortBufs &sb, cpx<T> *c, const c
 }
 
 template <typename T, int DIM>
-int interp_grad_dim(int ns, int M, const SortBufs &sb, cpx<T> *c, cpx<T> *dc, const cpx<T> *fw,
-                    const BinGeom &g, const KerParams<T> &kp, int ntr, long long nftot,
-                    long long dc_plane, const Launch &L)
-{
-    switch (ns) {
-#define CFA_CASE(N)                                                                          \
-    case N:                                                                                  \
-        return interp_grad_ns<T, DIM, N>(M, sb, c, dc, fw, g, kp, ntr, nftot, dc_plane, L);
-        CFA_NS_CASES(CFA_CASE)
-#undef CFA_CASE
-    }
-    return 1;
-}
-
-template <typename T, int DIM>
 int group_dim(int type, int ns, const void *items, int n, const long long *off,
               const KerParams<T> &kp, const int *bs, int maxsub, long long jobs_bound,
               const Launch &L)
@@ -254,8 +242,7 @@ void group_item_fill(void *dst, const BinGeom &g, const SortBufs &sb, int maxsub
 
 }  // namespace cfa
 
-// Spread side (spread kernels and plan groups) and interp side (interp and
-// coordinate-derivative kernels) instantiate in separate translation units so
+// Spread side (spread kernels and plan groups) and interp side instantiate in separate translation units so
 // the two halves of one precision and dimension compile in parallel.
 #define CFA_INSTANTIATE_SPREAD(T, DIM)                                                        \
     template int cfa::group_dim<T, DIM>(int, int, const void *, int, const long long *,      \
@@ -270,10 +257,6 @@ void group_item_fill(void *dst, const BinGeom &g, const SortBufs &sb, int maxsub
                                          int, long long, int, long long, const cfa::Launch &);
 
 #define CFA_INSTANTIATE_INTERP(T, DIM)                                                        \
-    template int cfa::interp_grad_dim<T, DIM>(int, int, const cfa::SortBufs &, cfa::cpx<T> *, \
-                                              cfa::cpx<T> *, const cfa::cpx<T> *,            \
-                                              const cfa::BinGeom &, const cfa::KerParams<T> &, \
-                                              int, long long, long long, const cfa::Launch &); \
     template int cfa::interp_dim<T, DIM>(int, int, int, const cfa::SortBufs &, cfa::cpx<T> *, \
                                          const cfa::cpx<T> *, const cfa::BinGeom &,          \
                                          const cfa::KerParams<T> &, int, long long, int,     \

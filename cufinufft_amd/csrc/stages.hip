@@ -21,15 +21,6 @@ size_t group_item_bytes(bool dbl)
     return dbl ? sizeof(GroupItem<double>) : sizeof(GroupItem<float>);
 }
 
-size_t sm_tile_bytes(int dim, int ns, const int *bs, size_t elem)
-{
-    const int P = (ns + 1) / 2;
-    size_t cells = (size_t)(bs[0] + 2 * P);
-    if (dim > 1) cells *= (size_t)(bs[1] + 2 * P);
-    if (dim > 2) cells *= (size_t)(bs[2] + 2 * P);
-    return 2 * cells * elem;
-}
-
 size_t blk_lds_bytes(int dim, const int *ob, size_t elem, int maxsub, int fxwide)
 {
     size_t cells = (size_t)ob[0];
@@ -38,12 +29,6 @@ size_t blk_lds_bytes(int dim, const int *ob, size_t elem, int maxsub, int fxwide
     (void)maxsub;
     const int acc = elem == 4 ? (fxwide ? 8 : 4) : 8;
     return blk_lds(cells, acc);
-}
-
-size_t spread_lds_bytes(int dim, int ns, const int *bs, size_t elem, int maxsub, int fxwide)
-{
-    const size_t cells = sm_tile_bytes(dim, ns, bs, 1) / 2;
-    return elem == 4 ? fx_spread_lds(cells, maxsub, fxwide, 4) : fx_spread_lds(cells, maxsub, 1, 8);
 }
 
 // Method 4, first half of setpts: count the copies of every point per
@@ -150,19 +135,6 @@ int run_interp(int method, int dim, int ns, int M, const SortBufs &sb, cpx<T> *c
 }
 
 template <typename T>
-int run_interp_grad(int dim, int ns, int M, const SortBufs &sb, cpx<T> *c, cpx<T> *dc,
-                    const cpx<T> *fw, const BinGeom &g, const KerParams<T> &kp, int ntr,
-                    long long nftot, long long dc_plane, const Launch &L)
-{
-    if (M == 0) return 0;
-    if (dim == 1)
-        return interp_grad_dim<T, 1>(ns, M, sb, c, dc, fw, g, kp, ntr, nftot, dc_plane, L);
-    if (dim == 2)
-        return interp_grad_dim<T, 2>(ns, M, sb, c, dc, fw, g, kp, ntr, nftot, dc_plane, L);
-    return interp_grad_dim<T, 3>(ns, M, sb, c, dc, fw, g, kp, ntr, nftot, dc_plane, L);
-}
-
-template <typename T>
 int run_deconvolve(int dim, const int *ms, const int *nf, const cpx<T> *fw, cpx<T> *fk,
                    const T *const *ker, int ntr, const Launch &L)
 {
@@ -186,20 +158,20 @@ int run_deconvolve(int dim, const int *ms, const int *nf, const cpx<T> *fw, cpx<
 
 template <typename T>
 int run_amplify(int dim, const int *ms, const int *nf, cpx<T> *fw, const cpx<T> *fk,
-                const T *const *ker, int ntr, const Launch &L)
+                const T *const *ker, int ntr, const Launch &L, int wax, int wsign)
 {
     const long long rows = (long long)nf[1] * nf[2] * ntr;
     const long long ntot = (long long)ms[0] * ms[1] * ms[2];
     const int grid = (int)(rows < (long long)L.ncu * 64 ? rows : (long long)L.ncu * 64);
     if (dim == 1)
         k_amplify<T, 1><<<grid, 256, 0, L.stream>>>(ms[0], ms[1], ms[2], nf[0], nf[1], nf[2], fw,
-                                                    fk, ker[0], ker[1], ker[2], rows, ntot);
+                                                    fk, ker[0], ker[1], ker[2], rows, ntot, wax, wsign);
     else if (dim == 2)
         k_amplify<T, 2><<<grid, 256, 0, L.stream>>>(ms[0], ms[1], ms[2], nf[0], nf[1], nf[2], fw,
-                                                    fk, ker[0], ker[1], ker[2], rows, ntot);
+                                                    fk, ker[0], ker[1], ker[2], rows, ntot, wax, wsign);
     else
         k_amplify<T, 3><<<grid, 256, 0, L.stream>>>(ms[0], ms[1], ms[2], nf[0], nf[1], nf[2], fw,
-                                                    fk, ker[0], ker[1], ker[2], rows, ntot);
+                                                    fk, ker[0], ker[1], ker[2], rows, ntot, wax, wsign);
     return ok_or_err();
 }
 
@@ -223,13 +195,10 @@ int run_zero(cpx<T> *p, long long n, const Launch &L)
     template int run_interp<T>(int, int, int, int, const SortBufs &, cpx<T> *,               \
                                const cpx<T> *, const BinGeom &, const KerParams<T> &, int,   \
                                long long, int, long long, const Launch &);                   \
-    template int run_interp_grad<T>(int, int, int, const SortBufs &, cpx<T> *, cpx<T> *,     \
-                                    const cpx<T> *, const BinGeom &, const KerParams<T> &,   \
-                                    int, long long, long long, const Launch &);              \
     template int run_deconvolve<T>(int, const int *, const int *, const cpx<T> *, cpx<T> *,  \
                                    const T *const *, int, const Launch &);                   \
     template int run_amplify<T>(int, const int *, const int *, cpx<T> *, const cpx<T> *,     \
-                                const T *const *, int, const Launch &);                      \
+                                const T *const *, int, const Launch &, int, int);            \
     template int run_zero<T>(cpx<T> *, long long, const Launch &);
 
 CFA_INST(float)

@@ -15,8 +15,9 @@ type 2, s = isign):
 
 * type 2, strengths: grad_fk = type-1 transform with sign -s of grad_c;
 * type 2, points:    grad_x_d[j] = Re(conj(grad_c[j]) * dc_j/dx_d), with dc
-  from execute_grad of the forward plan (one pass of the hand-written
-  interpolation kernel with the ES kernel and its derivative);
+  from execute_grad of the forward plan: dc_j/dx_d is the type-2 transform
+  of the modes weighted by i s k_d (weight applied in the amplify pass), so
+  the gradient carries the transform's own tolerance;
 * type 1, strengths: grad_c = h = type-2 transform with sign -s of grad_fk;
 * type 1, points:    grad_x_d[j] = Re(c_j * conj(dh_j/dx_d)), with h and dh
   from one execute_grad of the sign -s type-2 plan.

@@ -68,6 +68,23 @@ int cufinufftf_destroy(cufinufftf_plan plan);
 
 /* ---- Extensions (no counterpart in the reference ABI) ------------------ */
 
+/* makeplan with planning flags (0 gives makeplan exactly).
+ * CUFINUFFT_ACCURACY_FIRST: the reference's kernel width rule
+ * ns = ceil(-log10(tol/10)) (contrib/spreadinterp.cpp:40-50) leaves the
+ * transform's relative l2 error at 0.6-1.1, 1-2 and 1.5-3.2 times tol in 1,
+ * 2 and 3 dimensions (FP64, against direct sums, tol 1e-5 and 1e-9); this
+ * flag plans for tol * 2 / (dim + 2) instead (tol/2.5 in 3D: one more tap
+ * per axis at those tolerances), which measures 0.1-0.4 times tol.  For
+ * callers that need the requested tolerance met, at the cost of the wider
+ * kernel. */
+#define CUFINUFFT_ACCURACY_FIRST 1
+int cufinufft_makeplan_ext(int type, int dim, int *nmodes, int iflag, int ntransf,
+                           double tol, int maxbatchsize, cufinufft_plan *plan,
+                           cufinufft_opts *opts, int flags);
+int cufinufftf_makeplan_ext(int type, int dim, int *nmodes, int iflag, int ntransf,
+                            float tol, int maxbatchsize, cufinufftf_plan *plan,
+                            cufinufft_opts *opts, int flags);
+
 /* Stream for every launch and FFT of this plan (a hipStream_t; NULL is the
  * null stream, the default).  For callers that order the library with their
  * own streams, e.g. PyTorch's current stream or a HIP graph capture. */
@@ -101,10 +118,12 @@ int cufinufftf_group_destroy(cufinufftf_group group);
 /* Type-2 execute that also returns each output's derivatives with respect to
  * its point's coordinates (the differentiable form): c as in execute, and dc
  * with dim planes of M*ntransf complex values, dc[d*M*ntransf + t*M + j] =
- * d c[t*M + j] / d x_d[j] (x_0 = x, x_1 = y, x_2 = z).  One pass over the
- * interpolation taps with the ES kernel and its derivative.  For autograd
- * callers (gradients of losses with respect to point coordinates); returns 1
- * for type-1 plans. */
+ * d c[t*M + j] / d x_d[j] (x_0 = x, x_1 = y, x_2 = z).  Each derivative is
+ * the type-2 transform of the modes weighted by i * iflag * k_d (the weight
+ * is applied in the amplify pass), so it meets the plan's tolerance like the
+ * values; the cost is dim + 1 type-2 executes.  For autograd callers
+ * (gradients of losses with respect to point coordinates); returns 1 for
+ * type-1 and spread/interp-only plans. */
 int cufinufft_execute_grad(void *c, void *dc, void *fk, cufinufft_plan plan);
 int cufinufftf_execute_grad(void *c, void *dc, void *fk, cufinufftf_plan plan);
 

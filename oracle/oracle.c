@@ -116,12 +116,13 @@ int orc_setup_spreader_d(double eps, double upsampfac, int kerevalmeth,
     return ier;
 }
 
-/* SET_NF_TYPE12 for gpu_method 1/2: contrib/common.cpp:24-37.              */
-int orc_set_nf_type12(int ms, double upsampfac, int ns)
+/* SET_NF_TYPE12: contrib/common.cpp:24-37.  b is the output block size for
+   gpu_method 4 (nf becomes a multiple of it), 1 for the other methods.     */
+int orc_set_nf_type12(int ms, double upsampfac, int ns, int b)
 {
     int nf = (int)(upsampfac * ms);
     if (nf < 2 * ns) nf = 2 * ns;
-    if (nf < 2147483647) nf = orc_next235beven(nf, 1);
+    if (nf < 2147483647) nf = orc_next235beven(nf, b);
     return nf;
 }
 

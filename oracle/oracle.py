@@ -51,7 +51,7 @@ def _declare(L):
     L.orc_setup_spreader_f.restype = i
     L.orc_setup_spreader_d.argtypes = [d, d, i, ip, ctypes.POINTER(d), ctypes.POINTER(d), ctypes.POINTER(d)]
     L.orc_setup_spreader_d.restype = i
-    L.orc_set_nf_type12.argtypes = [i, d, i]
+    L.orc_set_nf_type12.argtypes = [i, d, i, i]
     L.orc_set_nf_type12.restype = i
     L.orc_rescale_f.argtypes = [i, p, i, p]
     L.orc_rescale_d.argtypes = [i, p, i, p]
@@ -116,8 +116,8 @@ def next235beven(n, b=1):
     return lib().orc_next235beven(int(n), int(b))
 
 
-def set_nf_type12(ms, ns, upsampfac=2.0):
-    return lib().orc_set_nf_type12(int(ms), float(upsampfac), int(ns))
+def set_nf_type12(ms, ns, upsampfac=2.0, b=1):
+    return lib().orc_set_nf_type12(int(ms), float(upsampfac), int(ns), int(b))
 
 
 def numbins(nf, bs, single):
@@ -264,14 +264,16 @@ class Plan:
     """Oracle plan: same parameters as the library's makeplan (types 1/2)."""
 
     def __init__(self, nufft_type, n_modes, tol, single, iflag=None, upsampfac=2.0,
-                 exact=True):
+                 exact=True, blocks=None):
         self.type = nufft_type
         self.ms = list(n_modes)
         self.dim = len(self.ms)
         self.single = single
         self.iflag = (1 if nufft_type == 1 else -1) if iflag is None else iflag
         self.sp = setup_spreader(tol, single, upsampfac)
-        self.nf = [set_nf_type12(m, self.sp.ns, upsampfac) for m in self.ms]
+        # blocks: the gpu_method 4 output block sizes (nf rounded to multiples)
+        bl = list(blocks) if blocks else [1] * self.dim
+        self.nf = [set_nf_type12(m, self.sp.ns, upsampfac, b) for m, b in zip(self.ms, bl)]
         self.kers = [fseries(n, self.sp) for n in self.nf]
         self.exact = exact
 

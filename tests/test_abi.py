@@ -89,8 +89,10 @@ def _mk(L, typ=1, dim=2, nm=(16, 16, 1), ntr=1, tol=1e-6, single=True, **kw):
     (dict(gpu_spreadinterponly=1, nm=(8, 8, 1)), 3),  # grid smaller than 2 ns
     (dict(gpu_binsizex=0), 1),                     # invalid bin size
     (dict(typ=2, gpu_method=4), 2),                # block method spreads only (type 1)
-    # block method: blocks must tile nf (32 here) and hold whole bins
-    (dict(dim=3, nm=(16, 16, 16), gpu_method=4, gpu_obinsizex=12, gpu_binsizex=4), 1),
+    # block method: nf is rounded to a multiple of the block, so blocks must
+    # be 2,3,5-smooth (14 is not: the reference's search would never end),
+    # and they must hold whole bins
+    (dict(dim=3, nm=(16, 16, 16), gpu_method=4, gpu_obinsizex=14, gpu_binsizex=7), 1),
     (dict(dim=3, nm=(16, 16, 16), gpu_method=4, gpu_obinsizex=8, gpu_binsizex=3), 1),
     # block of 64^3 cells: its LDS tile cannot fit
     (dict(dim=3, nm=(32, 32, 32), gpu_method=4, gpu_obinsizex=64, gpu_obinsizey=64,

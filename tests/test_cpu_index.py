@@ -1,0 +1,49 @@
+"""Host-side index checks of setpts and the SM spread (tests/cpu_index):
+the library's index helpers compiled for the CPU with AddressSanitizer and
+UndefinedBehaviorSanitizer, driven with the points of a workload.  Every sort
+destination must be hit once and the result sorted and stable; every bin,
+subproblem, tap window, round-robin slot, LDS byte and flushed grid offset
+must stay inside its allocation.  Small sizes here; the 1e8-point runs of
+the 3D type-1 workloads use the same binary with their sizes."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpu_index")
+
+
+@pytest.fixture(scope="module")
+def index_check():
+    if not (shutil.which("hipcc") or os.path.exists("/opt/rocm/bin/hipcc")):
+        pytest.skip("hipcc not available")
+    subprocess.check_call(["make", "-s", "-C", HERE, "index_check"])
+    return os.path.join(HERE, "index_check")
+
+
+CASES = [
+    # prec dim nf1 nf2 nf3 ns M bins maxsub tol dist
+    "f 3 64 64 64 6 300000 8 8 8 1024 1e-5 mixed",      # 32-bit cells, multi-chunk drains
+    "f 3 64 48 40 7 200000 16 8 8 64 1e-6 mixed",       # 64-bit cells, 256 threads
+    "f 3 48 48 48 7 200000 16 8 8 2048 1e-6 uniform",   # 64-bit cells, 1024 threads
+    "d 3 64 64 64 10 200000 8 8 8 1024 1e-9 mixed",
+    "d 3 40 40 40 11 100000 8 8 8 1024 4e-10 mixed",    # accuracy-first width
+    "f 2 512 512 1 7 400000 32 32 1 1024 1e-6 uniform",
+    "f 2 1024 1024 1 6 300000 64 64 1 8192 1e-5 mixed",
+    "d 1 8192 1 1 9 100000 1024 1 1 1024 1e-8 mixed",
+    "f 3 30 30 20 9 100000 8 8 8 1024 1e-5 mixed",      # sigma 1.25 widths
+    "d 3 30 30 30 15 50000 4 4 4 1024 1e-9 mixed",
+    # keys wide enough for two and three counting passes
+    "f 3 256 256 256 6 200000 4 4 4 1024 1e-5 mixed",
+    "f 3 512 512 512 6 100000 2 2 2 1024 1e-5 uniform",
+]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_index_check(index_check, case):
+    r = subprocess.run([index_check] + case.split(), capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "OK: 0 failed checks" in r.stdout

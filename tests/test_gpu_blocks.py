@@ -62,6 +62,9 @@ def test_spread_only(torch_dev, case):
 
 NU_CASES = [
     (2, np.float32, (256, 256), 100_000, 1e-5, (64, 64, 1), 1024, "uniform"),
+    # nf = 100 is no multiple of the 8-cell blocks: rounded up to 120 as the
+    # reference's method 4 does (contrib/common.cpp:31-32)
+    (3, np.float32, (50, 50, 50), 20_000, 1e-5, (8, 8, 8), 1024, "uniform"),
     (3, np.float32, (32, 32, 32), 50_000, 1e-6, (16, 16, 8), 32, "clustered"),
     (3, np.float32, (32, 32, 32), 50_000, 1e-5, (32, 16, 8), 4096, "clustered"),
     (3, np.float64, (32, 24, 28), 30_000, 1e-9, (16, 16, 12), 256, "clustered"),  # nf 64x48x60
@@ -79,7 +82,8 @@ def test_type1(torch_dev, case):
                            **_ob(ob))
     kidx = np.random.default_rng(8).choice(int(np.prod(nm)), 800, replace=False)
     direct = O.dirft_type1([p.astype(np.float64) for p in pts], c, list(nm), +1, kidx)
-    P = O.Plan(1, list(nm), tol, single=dtype == np.float32)
+    P = O.Plan(1, list(nm), tol, single=dtype == np.float32, blocks=ob[:dim])
+    assert tuple(plan.info()["nf"][:dim]) == tuple(P.nf)
     ofk = P.type1(pts, c)
     oerr = rel_l2(ofk.ravel()[kidx], direct)
     err = rel_l2(fk.ravel()[kidx], direct)

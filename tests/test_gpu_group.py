@@ -135,3 +135,37 @@ def test_plan_execute_under_torch_graph_capture(torch_dev):
         torch.cuda.synchronize()
         assert rel_l2(fk.cpu().numpy(), ref.cpu().numpy()) < 1e-5
         _ = torch.ones(10, device=torch_dev).sum().item()
+
+
+def test_group_mixed_tolerances(torch_dev):
+    """Members of one kernel width may differ in tol and maxsubprobsize: here
+    tol 5e-5 with 1024-point subproblems next to tol 1e-5 with 512-point ones
+    (both ns = 6).  The group's fixed-point tile must be sized for the
+    strictest member's 32-bit bound, so the 5e-5 member's 537..1024-point
+    jobs take 64-bit cells with room for them in the LDS."""
+    import torch
+    from cufinufft_amd import cufinufft
+    from cufinufft_amd.cufinufft import cufinufft_group
+    print("group mixed tolerances", flush=True)
+    dtype, N = np.float32, (48, 40)
+    ms = []
+    for i, (tol, maxsub) in enumerate(((5e-5, 1024), (1e-5, 512), (5e-5, 1024))):
+        M = 30_000
+        pts = G.clustered_pts(M, 2, dtype, 70 + i, ncl=2, sigma=0.05)
+        plan = cufinufft(1, N[::-1], eps=tol, dtype=dtype, maxbatchsize=1, gpu_method=2,
+                         gpu_binsizex=16, gpu_binsizey=16, gpu_maxsubprobsize=maxsub)
+        plan.set_pts(*[G.to_dev(p, torch_dev) for p in pts[::-1]])
+        c = G.to_dev(G.strengths(M, dtype, 70 + i), torch_dev)
+        fk = torch.zeros(N[::-1], dtype=c.dtype, device=torch_dev)
+        ms.append(dict(plan=plan, c=c, fk=fk, pts=pts, tol=tol))
+    refs = [individual(m, 1) for m in ms]
+    grp = cufinufft_group([m["plan"] for m in ms], [m["c"] for m in ms], [m["fk"] for m in ms])
+    for m in ms:
+        m["fk"].zero_()
+    grp.execute()
+    torch.cuda.synchronize()
+    for m, r in zip(ms, refs):
+        got = m["fk"].cpu().numpy()
+        assert rel_l2(got, r) < 1e-5
+        P = O.Plan(1, list(N), m["tol"], single=True)
+        assert rel_l2(got, P.type1(m["pts"], m["c"].cpu().numpy())) < m["tol"] / 5

@@ -116,36 +116,32 @@ def test_spread_only(torch_dev, case, method):
 
 
 WIDE_CASES = [
-    # dim, dtype, modes, M, tol, bins: tiles over 80 KiB of LDS
-    (3, np.float32, (64, 64, 32), 40000, 1e-3, (32, 32, 8)),  # 32-bit fixed-point tile
-    (3, np.float32, (24, 20, 28), 20000, 1e-6, (16, 8, 8)),   # 64-bit tile
+    # dim, dtype, modes, M, tol, bins: 64-bit cells over half the 160 KiB LDS
+    (3, np.float32, (24, 20, 28), 20000, 1e-6, (16, 8, 8)),
     (3, np.float64, (24, 24, 24), 12000, 1e-9, (8, 8, 8)),
-    (2, np.float32, (128, 96), 20000, 1e-4, (128, 64)),
     (1, np.float64, (8192,), 12000, 1e-8, (6000,)),
 ]
 
 
-def _spread_lds(dim, ns, bins, elem, maxsub, wide):
-    """LDS bytes of one SM spread workgroup: the padded fixed-point tile, the
-    staged strengths and the reduction scratch."""
+def _spread_lds(dim, ns, bins, elem, bt, acc, drain):
+    """LDS bytes of one SM spread workgroup of bt threads: the padded
+    fixed-point tile (acc bytes per re / im cell), the staged chunk (start
+    cell, dim offsets, complex strength), 64 counters, per-wave reduction
+    values and, for 32-bit cells with multi-chunk jobs, the FP32 drain planes."""
     P = (ns + 1) // 2
     cells = int(np.prod([b + 2 * P for b in bins[:dim]]))
-    return cells * (16 if wide else 8) + maxsub * 2 * elem + 16 * elem
+    return (cells * 2 * acc + bt * (4 + (dim + 2) * elem) + 256 + bt // 64 * elem +
+            (cells * 8 if drain else 0))
 
 
 @pytest.mark.parametrize("maxsub", [1024, 2048])
 @pytest.mark.parametrize("case", WIDE_CASES,
                          ids=lambda c: f"{c[0]}d-{np.dtype(c[1]).name}-{c[4]:g}")
 def test_spread_only_wide_workgroups(torch_dev, case, maxsub):
-    """Large tiles (over half the 160 KiB LDS) with subproblems of >= 1024
-    points spread with 1024-thread workgroups; on clustered points the
-    hot bins fill whole subproblems, and at maxsub 2048 each thread takes two
-    points.  Bars: as for the 256-thread spread where the tile is 64-bit; with
-    the 32-bit tile (4 * maxsub * 2^-30 <= tol/5) each tap rounds to 2^-30 of
-    the job's strength mass, so rel-l2 is held to tol/5; that rounding is
-    unbiased noise e, whose least-squares scale error |<e, ref>| / |ref|^2 is
-    about rel-l2 / sqrt(cells it touches), so the scale bar adds four times
-    that to the FP32 kernel-value level."""
+    """64-bit tiles over half the LDS with subproblems of >= 1024 points
+    spread with 1024-thread workgroups (one workgroup per CU either way);
+    on clustered points the hot bins fill whole subproblems, which stage in
+    one or two chunks of 1024.  Bars as for the 256-thread spread."""
     dim, dtype, nm, M, tol, bins = case
     _flush(f"spread-only wide {case} maxsub {maxsub}")
     pts = G.clustered_pts(M, dim, dtype, seed=8, ncl=2, sigma=0.3)
@@ -157,42 +153,50 @@ def test_spread_only_wide_workgroups(torch_dev, case, maxsub):
     # some bin holds more than one full subproblem
     assert plan.sort_arrays()["binsize"].max() > maxsub
     sp = O.setup_spreader(tol, dtype == np.float32)
-    u32 = dtype == np.float32 and 4 * maxsub * 2.0 ** -30 <= tol / 5
     elem = np.dtype(dtype).itemsize
-    lds = _spread_lds(dim, sp.ns, bins, elem, maxsub, not u32)
-    assert 160 * 1024 // 2 < lds <= 160 * 1024
+    assert _spread_lds(dim, sp.ns, bins, elem, 256, 8, False) > 160 * 1024 // 2
+    assert _spread_lds(dim, sp.ns, bins, elem, 1024, 8, False) <= 160 * 1024 - 64
     ref = O.spread(pts, c, list(nm), sp)
-    if dtype == np.float64:
-        bar = 1e-12
-    else:
-        bar = tol / 5 if u32 else 8e-7
-    err = rel_l2(fw, ref)
-    assert err < bar
-    sbar = 3e-7 if dtype == np.float32 else 1e-12
-    if u32:
-        sbar += 4 * err / np.sqrt(np.count_nonzero(ref))
-    assert scale_err(fw, ref) < sbar
+    bar = 1e-12 if dtype == np.float64 else 8e-7
+    assert rel_l2(fw, ref) < bar
+    assert scale_err(fw, ref) < (3e-7 if dtype == np.float32 else 1e-12)
 
 
-def test_spread_only_mixed_width(torch_dev):
-    """One FP32 plan whose jobs take both tile widths: at tol 1e-5 the 32-bit
-    tile holds jobs of up to 536 points (4 * n * 2^-30 <= tol/5), so with
-    1024-point subproblems the sparse bins of the uniform half accumulate in
-    32 bits and the full subproblems of the clusters in 64.  Bars as for the
-    32-bit tile."""
-    dim, dtype, nm, M, tol = 3, np.float32, (40, 36, 32), 40000, 1e-5
-    _flush("spread-only mixed width")
+CHUNK_CASES = [
+    # dim, modes, M, tol, bins, maxsub: FP32 32-bit cells (the 32-bit bound
+    # floor(tol/5 / 2^-28) >= 512 points), chunks of 512 points
+    (3, (40, 36, 32), 40000, 1e-5, (8, 8, 8), 4096),    # bound 536, up to 8 chunks
+    (3, (64, 64, 32), 40000, 1e-3, (16, 16, 8), 2048),
+    (2, (128, 96), 20000, 1e-4, (64, 64), 8192),
+    (1, (4096,), 20000, 1e-5, (1024,), 4096),
+]
+
+
+@pytest.mark.parametrize("case", CHUNK_CASES, ids=lambda c: f"{c[0]}d-{c[3]:g}-ms{c[5]}")
+def test_spread_only_chunked_32bit(torch_dev, case):
+    """FP32 plans whose tolerance admits 32-bit fixed point for a 512-point
+    chunk accumulate their jobs in such chunks, each under its own scale
+    (2^30 over the chunk's strength mass), drained into FP32 planes between
+    chunks.  Half uniform, half clustered points, so sparse bins take one
+    chunk and the cluster bins many.  Bars: each tap rounds to 2^-30 of a
+    chunk's strength mass, so rel-l2 is held to tol/5; that rounding is
+    unbiased noise e, whose least-squares scale error |<e, ref>| / |ref|^2 is
+    about rel-l2 / sqrt(cells it touches), so the scale bar adds four times
+    that to the FP32 kernel-value level."""
+    dim, nm, M, tol, bins, maxsub = case
+    dtype = np.float32
+    _flush(f"spread-only chunked {case}")
+    assert np.floor(tol / 5 / (4 * 2.0 ** -30)) >= 512
     u = G.uniform_pts(M // 2, dim, dtype, seed=9)
     cl = G.clustered_pts(M // 2, dim, dtype, seed=9, ncl=2, sigma=0.2)
     pts = [np.concatenate([a, b]) for a, b in zip(u, cl)]
     c = G.strengths(M, dtype, seed=9)
+    b3 = tuple(bins) + (1,) * (3 - dim)
     fw, plan = G.run_type1(torch_dev, pts, c, nm, tol, dtype, gpu_method=2,
-                           gpu_spreadinterponly=1, gpu_maxsubprobsize=1024, gpu_binsizex=8,
-                           gpu_binsizey=8, gpu_binsizez=8)
+                           gpu_spreadinterponly=1, gpu_maxsubprobsize=maxsub,
+                           gpu_binsizex=b3[0], gpu_binsizey=b3[1], gpu_binsizez=b3[2])
     bsz = plan.sort_arrays()["binsize"]
-    fx32max = int(np.floor(tol / 5 / (4 * 2.0 ** -30)))
-    assert fx32max == 536
-    assert np.any((bsz > 0) & (bsz <= fx32max)) and bsz.max() > 1024
+    assert np.any((bsz > 0) & (bsz <= 512)) and bsz.max() > 2 * 512
     sp = O.setup_spreader(tol, True)
     ref = O.spread(pts, c, list(nm), sp)
     err = rel_l2(fw, ref)
@@ -346,10 +350,10 @@ def test_dynamic_job_queues_repeated(torch_dev, nufft_type):
 
 @pytest.mark.parametrize("maxsub", [32, 1024])
 def test_type1_fp32_fixed_point_width(torch_dev, maxsub):
-    """FP32 SM spread at tol 1e-6 on clustered points: maxsub 32 keeps the
-    32-bit fixed-point tile (4 * 32 * 2^-30 <= tol/5); at maxsub 1024 jobs of
-    more than 53 points take the 64-bit tile (a 32-bit tile would add ~2e-6
-    here, CPU emulation), smaller ones stay 32-bit."""
+    """FP32 SM spread at tol 1e-6 on clustered points: the 32-bit bound is 53
+    points, below one workgroup-wide chunk, so the plan accumulates in 64-bit
+    cells whatever the subproblem size (a 32-bit tile over 1024 points would
+    add ~2e-6 here, CPU emulation)."""
     case = (3, np.float32, (32, 32, 32), 50_000, 1e-6, "clustered")
     dim, dtype, nm, M, tol, dist = case
     _flush(f"fixed-point width {case} maxsub {maxsub}")

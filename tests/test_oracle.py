@@ -40,6 +40,28 @@ def test_next235beven_bruteforce():
         assert O.next235beven(n) == _smooth_even_bruteforce(n), n
 
 
+def test_next235beven_block_multiple_bruteforce():
+    # method 4 rounds nf up to an even smooth multiple of the output block
+    # (contrib/common.cpp:31-32): brute force over n and smooth block sizes
+    for b in (1, 2, 3, 4, 5, 6, 8, 10, 12, 16, 24, 32):
+        for n in list(range(0, 400)) + [1001, 4097]:
+            m = max(n, 2)
+            m += m % 2
+            while True:
+                r = m
+                for p in (2, 3, 5):
+                    while r % p == 0:
+                        r //= p
+                if r == 1 and m % b == 0:
+                    break
+                m += 2
+            want = 2 if n <= 2 else m
+            assert O.next235beven(n, b) == want, (n, b)
+    # the example the reference's rule changes: N=50, ns=6, 8^3 blocks
+    assert O.set_nf_type12(50, 6) == 100
+    assert O.set_nf_type12(50, 6, 2.0, 8) == 120
+
+
 def test_nf_table():
     # fine-grid sizes of the workload shapes (sigma = 2): 2N is already smooth
     for N, nf in ((256, 512), (128, 256), (512, 1024), (320, 640), (100, 200)):
