@@ -217,8 +217,42 @@ static void fseries(int nf, const Spread<T> &sp, std::vector<double> &out)
 // its own scale and drained to FP32 between chunks, when a workgroup-wide
 // chunk fits the bound, else 64-bit cells.  Method 4 takes 32 bits for the
 // whole job when its subproblems fit the bound (fx_wide = 0).
-static int fx32_max(double tol) { return fx32_bound(tol); }
-static int fx_wide(int maxsub, double tol) { return maxsub > fx32_max(tol) ? 1 : 0; }
+//
+// The rounding is white noise on the fine grid, while the spread signal's
+// spectrum is shaped by the kernel: deconvolution divides mode k by the
+// kernel's Fourier factor phi^(k), so the noise reaches the modes amplified
+// by rms_k phi^(0) / phi^(k) relative to the signal.  That rms is 1.1-3.2
+// for every sigma = 2 plan (largest in 3D at tol 1e-6, the calibration case
+// above) but ~250 (2D) and ~3000 (3D) at sigma = 1.25, whose passband reaches
+// further into the kernel's decay.  The bound is therefore divided by the
+// plan's amplification over kAmpRef = 3.25 when it exceeds that.
+constexpr double kAmpRef = 3.25;
+static double deconv_amp(int dim, const int *ms, const std::vector<double> *fser)
+{
+    double amp = 1.0;
+    for (int d = 0; d < dim; ++d) {
+        if (fser[d].empty()) continue;
+        const int h = ms[d] / 2;
+        double s = 0.0;
+        long long n = 0;
+        for (int k = -h; k <= (ms[d] - 1) / 2; ++k) {
+            const double r = fser[d][0] / fser[d][k < 0 ? -k : k];
+            s += r * r;
+            ++n;
+        }
+        amp *= std::sqrt(s / (double)n);
+    }
+    return amp;
+}
+static int fx32_max(double tol, double amp)
+{
+    const int n = fx32_bound(tol);
+    return amp > kAmpRef ? (int)((double)n * kAmpRef / amp) : n;
+}
+static int fx_wide(int maxsub, double tol, double amp)
+{
+    return maxsub > fx32_max(tol, amp) ? 1 : 0;
+}
 
 // ---------------------------------------------------------------------------
 // Piecewise-polynomial ES kernel for gpu_kerevalmeth = 1.  The reference
@@ -634,8 +668,13 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
     // Method: 0 picks SM when the tile fits the LDS, GM-sort otherwise.  The
     // spread tile accumulates in FP64 (FP64 plans) or fixed point (FP32
     // plans); the interp tile holds T.
-    fxwide = fx_wide(opts.gpu_maxsubprobsize, (double)tol);
-    fx32max = sizeof(T) == 4 ? fx32_max((double)tol) : 0;
+    // Fourier-series factors (FP64 on the host; uploaded below), which also
+    // set the fixed-point noise amplification of the type-1 deconvolution
+    if (!opts.gpu_spreadinterponly)
+        for (int d = 0; d < dim; ++d) fseries<T>(nf[d], sp, fser[d]);
+    const double amp = deconv_amp(dim, ms, fser);
+    fxwide = fx_wide(opts.gpu_maxsubprobsize, (double)tol, amp);
+    fx32max = sizeof(T) == 4 ? fx32_max((double)tol, amp) : 0;
     int bsv[3] = {opts.gpu_binsizex, opts.gpu_binsizey, opts.gpu_binsizez};
     auto tile_lds = [&]() {
         return type == 1 ? spread_cfg(dim, sp.ns, bsv, sizeof(T), fx32max,
@@ -697,7 +736,6 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
     // Fourier-series factors (FP64 on the host, stored in T on the device).
     if (!opts.gpu_spreadinterponly) {
         for (int d = 0; d < dim; ++d) {
-            fseries<T>(nf[d], sp, fser[d]);
             std::vector<T> h(fser[d].begin(), fser[d].end());
             if (dmalloc(&fwker[d], h.size())) return kErrHip;
             if (hipMemcpy(fwker[d], h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) !=

@@ -167,8 +167,8 @@ CHUNK_CASES = [
     # floor(tol/5 / 2^-28) >= 512 points), chunks of 512 points
     (3, (40, 36, 32), 40000, 1e-5, (8, 8, 8), 4096),    # bound 536, up to 8 chunks
     (3, (64, 64, 32), 40000, 1e-3, (16, 16, 8), 2048),
-    (2, (128, 96), 20000, 1e-4, (64, 64), 8192),
-    (1, (4096,), 20000, 1e-5, (1024,), 4096),
+    (2, (128, 96), 20000, 1e-4, (16, 16), 8192),
+    (1, (4096,), 20000, 1e-5, (64,), 4096),
 ]
 
 

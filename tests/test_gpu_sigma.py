@@ -5,7 +5,18 @@ oracle's sigma path and the direct sums, in 2D and 3D, FP32 and FP64, type 1
 and type 2.  The wider kernel (ns 9 at tol 1e-5, 15 at 1e-9) needs bins small
 enough for its tile to fit the LDS.  Bars: within tol/5 of the oracle
 (2e-6 for FP32, whose kernel values cost ~5e-7 in 3D) and rel-l2 against the
-direct sum within max(2 tol, 1.25 x the oracle's own error)."""
+direct sum within max(2 tol, 1.25 x the oracle's own error).
+
+At sigma = 1.25 the passband reaches far into the kernel's decay, so the
+deconvolution factors 1/phi^(k) grow to ~1e2 (2D) and ~1e3 (3D) at the band
+edge: the fine grid holds the modes amplified by them, and every FP32
+rounding on that grid (the FFT, the interp sums) reaches the result
+amplified alike.  The oracle computes in FP64, so an FP32 plan's distance
+from it is bounded by the same rounding, measured on the CPU: the oracle
+pipeline with its FFT done in complex64 (``_fft32_dev``) deviates by 1.4e-5
+(3D type 2), against 1e-7 at sigma = 2.  The FP32 bar is three times that
+deviation (the device also rounds its spread / interp sums in FP32), or
+2e-6, whichever is larger; the direct-sum bar is unchanged."""
 import numpy as np
 import pytest
 
@@ -22,6 +33,27 @@ CASES = [
     (3, np.float32, (24, 20, 16), 20000, 1e-5, (8, 8, 8)),
     (3, np.float64, (16, 16, 12), 8000, 1e-9, (4, 4, 4)),
 ]
+
+
+def _fft32_dev(P, kind, pts, data):
+    """rel-l2 distance between the oracle and the oracle with a complex64
+    FFT (the FP32 rounding of the amplified fine grid alone)."""
+    import scipy.fft as sfft
+
+    def fft32(fw):
+        a = fw.astype(np.complex64)
+        r = sfft.ifftn(a, norm="forward") if P.iflag >= 0 else sfft.fftn(a)
+        return r.astype(np.complex128)
+
+    if kind == 1:
+        fw = O.spread(pts, data, P.nf, P.sp, P.exact)
+        ref = O.deconvolve(O._fft(fw, P.iflag, 1), P.ms, P.nf, P.kers)
+        got = O.deconvolve(fft32(fw), P.ms, P.nf, P.kers)
+    else:
+        fw = O.amplify(data, P.ms, P.nf, P.kers)
+        ref = O.interp(pts, O._fft(fw, P.iflag, 1), P.nf, P.sp, P.exact)
+        got = O.interp(pts, fft32(fw), P.nf, P.sp, P.exact)
+    return rel_l2(got, ref)
 
 
 def _opts(bins):
@@ -48,7 +80,10 @@ def test_sigma125_type1(torch_dev, case):
     err = rel_l2(fk.ravel()[kidx], direct)
     print(f"   gpu err {err:.3e} oracle err {oerr:.3e}", flush=True)
     assert err <= max(2 * tol, 1.25 * oerr)
-    assert rel_l2(fk, ofk) <= max(tol / 5, 2e-6 if single else 0.0)
+    bar = max(tol / 5, 2e-6, 3 * _fft32_dev(P, 1, pts, c)) if single else tol / 5
+    dev = rel_l2(fk, ofk)
+    print(f"   vs oracle {dev:.3e} (bar {bar:.3e})", flush=True)
+    assert dev <= bar
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}d-{np.dtype(c[1]).name}")
@@ -67,4 +102,7 @@ def test_sigma125_type2(torch_dev, case):
     err = rel_l2(c[pidx], direct)
     print(f"   gpu err {err:.3e} oracle err {oerr:.3e}", flush=True)
     assert err <= max(2 * tol, 1.25 * oerr)
-    assert rel_l2(c, oc) <= max(tol / 5, 2e-6 if single else 0.0)
+    bar = max(tol / 5, 2e-6, 3 * _fft32_dev(P, 2, pts, fk)) if single else tol / 5
+    dev = rel_l2(c, oc)
+    print(f"   vs oracle {dev:.3e} (bar {bar:.3e})", flush=True)
+    assert dev <= bar
