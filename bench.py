@@ -173,6 +173,23 @@ def _kernel_matches(kern, tmpl):
             int(args[ints[0]]) == dim and int(args[ints[0] + 1]) == ns)
 
 
+def profiled_opts(workload):
+    """Option sets of the committed rocprofv3 summaries (TRAFFIC records in
+    profiles/) for this workload."""
+    root = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+    out = []
+    if not os.path.isdir(root):
+        return out
+    for fn in sorted(os.listdir(root)):
+        if fn.endswith(".md"):
+            for line in open(os.path.join(root, fn)):
+                if line.startswith("TRAFFIC "):
+                    rec = json.loads(line[len("TRAFFIC "):])
+                    if rec.get("workload") == workload and rec.get("opts") not in out:
+                        out.append(rec["opts"])
+    return out
+
+
 def measured_traffic(workload, tmpl, opts):
     """HBM bytes per launch of the kernel for this workload and options, from
     the TRAFFIC records of the committed rocprofv3 summaries in profiles/
@@ -544,6 +561,22 @@ def measure(w, wname, dev, dpts, din, sel, ref, a, accuracy_first=False):
         log(f"[bench] {wname}: tuned choice {best_opts}; re-timing")
         best = run_plan(w, dev, dpts, din, best_opts, a.steps, a.warmup, maxbatch,
                         accuracy_first=accuracy_first)
+        # the tuner's top candidates differ by a few percent at most; an
+        # option set with a committed HBM-counter profile that times within
+        # 2% of its choice is taken instead, so that the roofline's traffic
+        # figure belongs to the options actually timed
+        for po in profiled_opts(wname):
+            if _layout(po) == _layout(best_opts):
+                break
+            try:
+                r = run_plan(w, dev, dpts, din, po, a.steps, a.warmup, maxbatch,
+                             accuracy_first=accuracy_first)
+            except RuntimeError:
+                continue
+            log(f"[bench] {wname}: profiled option set {po}: {r['t_exec'] * 1e3:.3f} ms/exec")
+            if r["t_exec"] <= 1.02 * best["t_exec"]:
+                best, best_opts = r, po
+                break
         # at equal kernel width the plain options are a candidate too; an
         # accuracy-first plan is compared at its own (wider) width only
         if plain["t_exec"] < best["t_exec"] and not accuracy_first:

@@ -90,6 +90,9 @@ def sigma125_candidates(opts, dim):
     their largest side (down to 4 cells) until the wider kernel's tile fits:
     the sequence of sets to try, largest tile first (method 2 only)."""
     base = dict(opts, upsampfac=1.25)
+    # the polynomial kernel is planned for sigma = 2 only (makeplan returns
+    # 8 otherwise, as the reference does)
+    base.pop("gpu_kerevalmeth", None)
     if base.get("gpu_method", 2) != 2:
         return [base]
     b = [base.get("gpu_binsizex", 16 if dim > 1 else 1024),

@@ -49,7 +49,21 @@ struct KerParams {
     // selects 64-bit block tiles when its subproblems exceed it
     int fxwide;
     int fx32max;
+    // beta * log2(e) as a sum bl2e + bl2e_lo of two T values (the FP64
+    // product split): FP32 kernels evaluate e^(beta s) as 2^(beta log2(e) s)
+    // with the hardware exp2, forming the exponent as
+    // fma(bl2e, s, bl2e_lo * s) so that it carries one rounding of its own
+    // and none of the constant (whose rounding, shared by every tap, would
+    // scale the whole grid by up to ~1e-6)
+    T bl2e;
+    T bl2e_lo;
 };
+
+// 2^(beta log2(e) s) in FP32 (see KerParams::bl2e).
+__device__ __forceinline__ float exp_beta(float s, const KerParams<float> &k)
+{
+    return __builtin_amdgcn_exp2f(fmaf(k.bl2e, s, k.bl2e_lo * s));
+}
 
 // Points as the execute kernels read them, in bin-sorted order: idx[i] is
 // the caller's index of sorted point i (for strengths / outputs) and
@@ -273,10 +287,18 @@ __device__ __forceinline__ int point_bin(const T *__restrict__ x, const T *__res
 // ES kernel phi(x) = exp(beta sqrt(1 - c x^2)) for |x| < ns/2, else 0
 // (src/cuspreadinterp.h:6-16).  The argument of sqrt is floored at 0 so that
 // rounding at |x| -> ns/2 cannot produce NaN.
+//
+// FP32 evaluates the exponential with the hardware exp2 (v_exp_f32, within
+// 0.8 ulp measured; the library's expf is ~12 VALU instructions more per
+// tap) on an exponent formed without rounding the constant (exp_beta).  The
+// square root stays the correctly rounded sqrtf: its error is multiplied by
+// beta (~14-20) in the exponent, and the hardware v_sqrt_f32 (within 0.9 ulp,
+// not correctly rounded) moved the least-squares scale of a 3D FP32 spread
+// by 4e-7.
 __device__ __forceinline__ float es_eval(float x, const KerParams<float> &k)
 {
     const float a = fmaxf(1.0f - k.c * x * x, 0.0f);
-    return fabsf(x) < k.hw ? expf(k.beta * sqrtf(a)) : 0.0f;
+    return fabsf(x) < k.hw ? exp_beta(sqrtf(a), k) : 0.0f;
 }
 __device__ __forceinline__ double es_eval(double x, const KerParams<double> &k)
 {
@@ -293,14 +315,19 @@ CFA_HD double tap_pos(double v, int n)
 
 // Horner evaluation of the ns per-tap polynomials at z (all taps of one
 // point share z); coefficient addresses are lane-uniform.
+// The table is read through the constant address space: it is read-only for
+// the whole launch and every lane reads the same address, so the loads are
+// scalar (s_load into SGPRs) instead of a chain of vector loads per degree.
 template <typename T, int NS>
 __device__ __forceinline__ void horner_taps(T z, const T *__restrict__ hc, int deg, T *ker)
 {
+    using CT = const __attribute__((address_space(4))) T;
+    CT *h = (CT *)hc;
 #pragma unroll
-    for (int i = 0; i < NS; ++i) ker[i] = hc[deg * NS + i];
+    for (int i = 0; i < NS; ++i) ker[i] = h[deg * NS + i];
     for (int k = deg - 1; k >= 0; --k) {
 #pragma unroll
-        for (int i = 0; i < NS; ++i) ker[i] = ker[i] * z + hc[k * NS + i];
+        for (int i = 0; i < NS; ++i) ker[i] = ker[i] * z + h[k * NS + i];
     }
 }
 
@@ -800,6 +827,11 @@ __device__ __forceinline__ T block_sum256(T v, T *scratch)
 template <typename Acc> struct FxAcc;
 template <> struct FxAcc<int> {
     static constexpr int kBits = 30;
+    // Round to nearest even (rndne + cvt).  The one-instruction
+    // floor(v + 1/2) (v_cvt_rpi_i32_f32) rounds exact halves up: a bias of
+    // ~2^-25 of each tap's value, always of one sign, which n taps on one cell
+    // sum coherently against the sqrt(n) growth of the cell value -- measured
+    // 4e-6 relative on 2D points packed ~15000 to a cell, twice the tol/5 bar
     __device__ static int cvt(float v) { return __float2int_rn(v); }
     __device__ static void add(int *p, int v) { atomicAdd(p, v); }
 };
@@ -828,7 +860,10 @@ __device__ __forceinline__ void taps_from_offset(T x1, const KerParams<T> &kp, T
     for (int i = 0; i < NS; ++i) {
         const T t = x1 + (T)i;
         const T a = fmax(T(1) - kp.c * t * t, T(0));
-        ker[i] = fabs(t) < kp.hw ? exp(kp.beta * (sqrt(a) - T(1))) : T(0);
+        if constexpr (sizeof(T) == 4)
+            ker[i] = fabsf(t) < kp.hw ? exp_beta(sqrtf(a) - 1.0f, kp) : 0.0f;
+        else
+            ker[i] = fabs(t) < kp.hw ? exp(kp.beta * (sqrt(a) - T(1))) : T(0);
     }
 }
 
@@ -858,6 +893,41 @@ CFA_HD int rr_slot(const int *cnt, int nk, int key, int rank)
         slot += (ck < rank ? ck : rank) + (k < key && ck > rank ? 1 : 0);
     }
     return slot;
+}
+
+// rr_slot for the lanes of one wave (every lane of the wave calls it; key and
+// rank are those of the lane's point, anything for lanes without one).  The
+// counts are read once per lane: lane k < nk holds cnt[k].  With R the
+// largest count, rounds r < min(R, 64) are tabulated in lanes: lane r keeps
+// S(r) = sum_k min(cnt_k, r) (built as the running sum of the rounds' sizes)
+// and the ballot of {k : cnt_k > r}; a point of rank r < 64 then reads its
+// round's entries by lane shuffle and counts the residues before its own.
+__device__ __forceinline__ int rr_slot_wave(const int *cnt, int nk, int key, int rank)
+{
+    const int lane = threadIdx.x & 63;
+    const int ck = lane < nk ? cnt[lane] : 0;
+    int mx = ck;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) mx = max(mx, __shfl_xor(mx, d, 64));
+    const int R = mx < 64 ? mx : 64;
+    int myS = 0, runS = 0;
+    unsigned long long mym = 0;
+    for (int r = 0; r < R; ++r) {
+        const unsigned long long m = __ballot(ck > r);
+        if (lane == r) {
+            myS = runS;
+            mym = m;
+        }
+        runS += __popcll(m);
+    }
+    const int rr = rank < 64 ? rank : 0;
+    const int S = __shfl(myS, rr, 64);
+    const unsigned lo = __shfl((unsigned)mym, rr, 64);
+    const unsigned hi = __shfl((unsigned)(mym >> 32), rr, 64);
+    if (rank >= 64) return rr_slot(cnt, nk, key, rank);
+    const unsigned long long m = ((unsigned long long)hi << 32) | lo;
+    const unsigned long long below = key > 0 ? m & (~0ull >> (64 - key)) : 0ull;
+    return S + __popcll(below);
 }
 
 // SM spread of one (subproblem, transform) job (Spread_{2,3}d_Subprob,
@@ -971,8 +1041,12 @@ __device__ __forceinline__ void spread_sm_job(const SubprobJob &J, int M, const 
         } else {
             __syncthreads();
         }
+        // Slot from a per-wave table of the rounds: lane r holds S(r), the
+        // points of all rounds before r, and the mask of the residues with a
+        // point in round r (one ballot per round, rounds < 64); a rank of 64
+        // or more (a chunk crowded onto few residues) walks the counts.
+        const int slot = (tid & ~63) < nc ? rr_slot_wave(cnt, NK, key, tid < nc ? rank : 0) : 0;
         if (tid < nc) {
-            const int slot = rr_slot(cnt, NK, key, rank);
             rs[slot] = s;
 #pragma unroll
             for (int d = 0; d < DIM; ++d) rx[d * BT + slot] = x1[d];
@@ -1005,11 +1079,14 @@ __device__ __forceinline__ void spread_sm_job(const SubprobJob &J, int M, const 
         }
         if (DRAIN && ch + 1 < nch) {
             __syncthreads();
-            const double back = ldexp(eb, -sh);
+            // 32-bit cells convert exactly to FP32 up to 2^24 and to within
+            // one rounding beyond, like the FP32 factor: FP32 drains and
+            // flushes match the FP32 grid they feed
+            const float bf = (float)ldexp(eb, -sh);
             for (int p = tid; p < n; p += BT) {
                 const Acc ir = sre[p], ii = sim[p];
-                dre[p] += (float)((double)ir * back);
-                dimg[p] += (float)((double)ii * back);
+                dre[p] += (float)ir * bf;
+                dimg[p] += (float)ii * bf;
                 sre[p] = 0;
                 sim[p] = 0;
             }
@@ -1018,19 +1095,31 @@ __device__ __forceinline__ void spread_sm_job(const SubprobJob &J, int M, const 
     __syncthreads();
     const double back = ldexp(eb, -sh);
     cpx<T> *f = fw + (long long)J.t * nftot;
-    const bool drained = DRAIN && nch > 1;
-    for_tile_cells(tl, g, tid, BT, [&](int p, long long off) {
-        double vr = (double)sre[p] * back, vi = (double)sim[p] * back;
-        if (drained) {
-            vr += (double)dre[p];
-            vi += (double)dimg[p];
-        }
-        if (vr != 0.0 || vi != 0.0) {
-            cpx<T> *q = f + off;
-            gadd(&q->x, (T)vr);
-            gadd(&q->y, (T)vi);
-        }
-    });
+    if constexpr (DRAIN) {
+        const float bf = (float)back;
+        const bool drained = nch > 1;
+        for_tile_cells(tl, g, tid, BT, [&](int p, long long off) {
+            float vr = (float)sre[p] * bf, vi = (float)sim[p] * bf;
+            if (drained) {
+                vr += dre[p];
+                vi += dimg[p];
+            }
+            if (vr != 0.0f || vi != 0.0f) {
+                cpx<T> *q = f + off;
+                gadd(&q->x, (T)vr);
+                gadd(&q->y, (T)vi);
+            }
+        });
+    } else {
+        for_tile_cells(tl, g, tid, BT, [&](int p, long long off) {
+            const double vr = (double)sre[p] * back, vi = (double)sim[p] * back;
+            if (vr != 0.0 || vi != 0.0) {
+                cpx<T> *q = f + off;
+                gadd(&q->x, (T)vr);
+                gadd(&q->y, (T)vi);
+            }
+        });
+    }
     __syncthreads();
 }
 

@@ -413,7 +413,10 @@ struct Plan {
 
     KerParams<T> ker_params() const
     {
-        return KerParams<T>{sp.beta, sp.c, sp.hw, hcoef, hdeg, fxwide, fx32max};
+        const double bl2e = (double)sp.beta * 1.4426950408889634;
+        const T hi = (T)bl2e;
+        return KerParams<T>{sp.beta, sp.c,    sp.hw,   hcoef, hdeg,
+                            fxwide,  fx32max, hi,      (T)(bl2e - (double)hi)};
     }
 
     hipfftHandle fft_full = 0, fft_rem = 0;

@@ -172,8 +172,10 @@ int group_ns(int type, const GroupItem<T> *items, int n, const long long *off,
 
 }  // namespace
 
+#ifndef CFA_NS_CASES
 #define CFA_NS_CASES(F)                                                                      \
     F(2) F(3) F(4) F(5) F(6) F(7) F(8) F(9) F(10) F(11) F(12) F(13) F(14) F(15) F(16)
+#endif
 
 template <typename T, int DIM>
 int spread_dim(int method, int ns, int M, const SortBufs &sb, const cpx<T> *c, cpx<T> *fw,
