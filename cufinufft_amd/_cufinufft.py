@@ -191,6 +191,12 @@ for _suffix in ("", "f"):
     _f.argtypes = [c_void_p]
     _f.restype = c_int
 
+# setpts without allocation (FP32 plans): sort scratch from the library's
+# fixed device array; returns 13 beyond what is allocated
+_set_pts_noallocf = lib.cufinufftf_setpts_noalloc
+_set_pts_noallocf.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_void_p]
+_set_pts_noallocf.restype = c_int
+
 STAGES = ("sort", "grid_zero", "spread_interp", "fft", "deconv_amplify")
 
 EXPORTED = [
@@ -212,4 +218,5 @@ EXPORTED = [
     "cufinufft_group_execute", "cufinufftf_group_execute",
     "cufinufft_group_setstream", "cufinufftf_group_setstream",
     "cufinufft_group_destroy", "cufinufftf_group_destroy",
+    "cufinufftf_setpts_noalloc",
 ]

@@ -15,6 +15,7 @@ constexpr int kErrHip = 10;
 constexpr int kJobqWords = 9;  // SM job queue: one counter per XCD range + finished workgroups
 constexpr int kErrFft = 11;
 constexpr int kErrLds = 12;
+constexpr int kErrNoAlloc = 13;  // no-allocation entry: input beyond what is allocated
 // Dynamic LDS of one gfx950 workgroup: its 160 KiB less room for the
 // kernels' static job slot.
 constexpr size_t kMaxLdsBytes = 160 * 1024 - 64;

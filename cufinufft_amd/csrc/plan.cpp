@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -538,6 +539,8 @@ struct Plan {
     int makeplan(int type_, int dim_, const int *nmodes, int iflag_, int ntransf_, T tol,
                  int maxbatch_, const cufinufft_opts *o, int flags = 0);
     int setpts(int M_, const T *x, const T *y, const T *z);
+    int setpts_noalloc(int M_, const T *x, const T *y, const T *z, unsigned char *scr,
+                       size_t scr_bytes);
     int setpts_blocks();
     int cap_copies = -1;  // capacity of idx / pos in point copies (method 4)
     long long cap_counts = -1;          // capacity of sb.sort_counts
@@ -903,6 +906,65 @@ int Plan<T>::setpts(int M_, const T *x, const T *y, const T *z)
     if (e) return e;
     pts_set = true;
     return 0;
+}
+
+// setpts without allocation (cufinufftf_setpts_noalloc): the same bin sort,
+// with its transient buffers (the key ping-pong pair, the point records
+// between passes, the per-pass (digit, tile) counts and their scan partials)
+// carved from the caller-supplied device scratch instead of the plan's own.
+// The plan's lasting point arrays (sorted index, positions, subproblem map)
+// must already hold M points: an earlier setpts of at least M points sized
+// them.  Anything larger returns kErrNoAlloc and leaves the plan as it was.
+template <typename T>
+int Plan<T>::setpts_noalloc(int M_, const T *x, const T *y, const T *z, unsigned char *scr,
+                            size_t scr_bytes)
+{
+    if (M_ < 0) return kErrNdataNotValid;
+    if (M_ > 0 && (!x || (dim > 1 && !y) || (dim > 2 && !z))) return 1;
+    if (method == 4) return 2;  // block copies: their count is known only after counting
+    if (M_ > cap_M) return kErrNoAlloc;
+    const int maxsub = opts.gpu_maxsubprobsize;
+    const long long sub_need = nbins + (M_ + (long long)maxsub - 1) / maxsub;
+    const bool need_sub = method == 2;
+    if (need_sub && sub_need > cap_sub) return kErrNoAlloc;
+    SortBufs s2 = sb;
+    if (!(method == 1 && !opts.gpu_sort) && M_ > 0) {
+        const long long nkeys =
+            opts.gpu_sort == 2 ? nbins * (long long)geom.bs[0] * geom.bs[1] * geom.bs[2] : nbins;
+        const BinSortShape sh = bin_sort_shape(M_, nkeys);
+        const size_t rb = bin_sort_rec_bytes(dim, sizeof(T));
+        const long long parts = (sh.counts + kScanChunk - 1) / kScanChunk;
+        size_t off = 0;
+        auto take = [&](size_t bytes) -> unsigned char * {
+            unsigned char *p = scr + off;
+            off += (bytes + 255) / 256 * 256;
+            return p;
+        };
+        s2.skey = reinterpret_cast<int *>(take(sizeof(int) * (size_t)M_));
+        s2.skey_out = reinterpret_cast<int *>(take(sizeof(int) * (size_t)M_));
+        s2.rec[0] = sh.passes > 1 ? take(rb * (size_t)M_) : nullptr;
+        s2.rec[1] = sh.passes > 2 ? take(rb * (size_t)M_) : nullptr;
+        s2.sort_counts = reinterpret_cast<int *>(take(sizeof(int) * (size_t)sh.counts));
+        s2.sort_part = take(sizeof(int2) * (size_t)parts);
+        if (off > scr_bytes) return kErrNoAlloc;
+    }
+    DeviceGuard dg(opts.gpu_device_id);
+    M = M_;
+    kx = x;
+    ky = dim > 1 ? y : nullptr;
+    kz = dim > 2 ? z : nullptr;
+    sub_bound = sub_need;
+    int e;
+    tbegin();
+    if (method == 1 && !opts.gpu_sort) {
+        e = run_natural_order<T>(dim, geom, M, kx, ky, kz, s2, L);
+    } else {
+        e = run_bin_sort<T>(dim, geom, nbins, M, kx, ky, kz, maxsub, opts.gpu_sort == 2,
+                            need_sub, s2, L);
+    }
+    tend(kStageSort);
+    pts_set = e == 0;
+    return e;
 }
 
 template <typename T>
@@ -1398,6 +1460,48 @@ int cufinufftf_setpts(int M, float *x, float *y, float *z, int, float *, float *
                       cufinufftf_plan plan)
 {
     return plan ? plan->setpts(M, x, y, z) : 1;
+}
+
+}  // extern "C"
+
+// Device scratch of cufinufftf_setpts_noalloc: one fixed array per GPU, held
+// until the process ends.  Host code never uses the array's name as an
+// address (on the host it names a stand-in, not device memory): each device's
+// address is looked up once, with that device current, and cached.
+namespace {
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of the scratch on the current device (resolved on first use).
+int scratch_address(unsigned char **out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxScratchDevices)
+        return cfa::kErrHip;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return cfa::kErrHip;
+        g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int cufinufftf_setpts_noalloc(int M, float *x, float *y, float *z, cufinufftf_plan plan)
+{
+    if (!plan) return 1;
+    cfa::DeviceGuard dg(plan->opts.gpu_device_id);
+    unsigned char *scr = nullptr;
+    const int e = scratch_address(&scr);
+    if (e) return e;
+    return plan->setpts_noalloc(M, x, y, z, scr, kNoAllocScratchBytes);
 }
 
 int cufinufft_execute(void *c, void *fk, cufinufft_plan plan)

@@ -12,7 +12,8 @@
  *   3 ERR_SPREAD_BOX_SMALL, 4 ERR_SPREAD_PTS_OUT_RANGE, 5 ERR_SPREAD_ALLOC,
  *   6 ERR_SPREAD_DIR, 7 ERR_UPSAMPFAC_TOO_SMALL, 8 HORNER_WRONG_BETA,
  *   9 ERR_NDATA_NOTVALID, 10 HIP runtime error, 11 FFT library error,
- *   12 tile exceeds the 160 KiB LDS of one workgroup.
+ *   12 tile exceeds the 160 KiB LDS of one workgroup,
+ *   13 input beyond what is allocated (cufinufftf_setpts_noalloc only).
  */
 #ifndef CUFINUFFT_H
 #define CUFINUFFT_H
@@ -168,6 +169,22 @@ int cufinufftf_get_fseries(cufinufftf_plan plan, double *k1, double *k2, double 
  * z = 2(start - x) + ns - 1 and start = ceil(x - ns/2). */
 int cufinufft_get_kernel_poly(cufinufft_plan plan, int *deg, double *rel_err, double *coef);
 int cufinufftf_get_kernel_poly(cufinufftf_plan plan, int *deg, double *rel_err, double *coef);
+
+/* setpts that allocates nothing (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The same bin sort
+ * as cufinufftf_setpts, with its transient buffers (sort keys, point records
+ * between passes, per-pass digit counts) taken from one fixed 64 MiB device
+ * array of the library, held on each GPU it runs on until the process ends.
+ * The plan's own point arrays must already hold M points (an earlier
+ * cufinufftf_setpts of at least M points on this plan sized them); M beyond
+ * that, or a sort needing more than the 64 MiB (about 8 bytes per point plus
+ * 2 x (3 coordinates + index) bytes per point when the keys need more than
+ * one sort pass: ~2.6 million 3D points), returns 13 and changes nothing.
+ * Calls that overlap on one GPU share that array and race: serialise them
+ * (one stream, or synchronise between them). */
+int cufinufftf_setpts_noalloc(int M, float *x, float *y, float *z,
+                              cufinufftf_plan plan);
 
 #ifdef __cplusplus
 }
