@@ -125,11 +125,16 @@ k_sort_hist(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__
 // Pass scatter.  counts holds the exclusive scan of the histogram, so
 // counts[d * tiles + t] is where tile t's first item of digit d goes.  Wave w
 // of the tile owns items [t * tile + w * 1024, + 1024) in 16 rounds of 64:
-// first it counts its digits, then the four waves' bases follow from the
-// tile base in wave order, and each round ranks its lanes among equal digits
+// first it counts its digits; the tile's digit totals are scanned into the
+// tile-local start of every digit, and each wave's base within a digit
+// follows in wave order; then each round ranks its lanes among equal digits
 // with ballots (the lowest lane of a group advances the base), so equal keys
-// keep item order.  LAST writes the sorted key, index and rescaled positions
-// instead of the records.
+// keep item order.  The ranked items are first placed in LDS in their sorted
+// order inside the tile (key and input position), and the tile is written
+// out from there: consecutive threads take consecutive sorted items, whose
+// destinations are consecutive within each digit's run, so the records of a
+// run leave as whole cache lines instead of one line per lane.  LAST writes
+// the sorted key, index and rescaled positions instead of the records.
 template <typename T, int DIM, bool FINE, bool FIRST, bool LAST>
 __global__ void __launch_bounds__(256)
 k_sort_scatter(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__restrict__ z,
@@ -138,14 +143,21 @@ k_sort_scatter(int M, const T *__restrict__ x, const T *__restrict__ y, const T 
                int *__restrict__ kout, SortRec<T, DIM> *__restrict__ rout,
                int *__restrict__ idx, double *__restrict__ pos, long long stride)
 {
-    __shared__ int wb[kSortWaves][1 << kMaxDigitBits];
+    constexpr int kMaxDigits = 1 << kMaxDigitBits;
+    constexpr int kPer = kMaxDigits / 256;  // digits per thread in the tile scan
+    __shared__ int wb[kSortWaves][kMaxDigits];
+    __shared__ int dl[kMaxDigits];  // tile-local start of each digit's items
+    __shared__ int gd[kMaxDigits];  // global start of them
+    __shared__ int sk[kSortTile];   // the tile's keys in sorted order
+    __shared__ int sj[kSortTile];   // and their input positions
     const int nd = 1 << dbits;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int d = threadIdx.x; d < nd; d += 256)
 #pragma unroll
         for (int q = 0; q < kSortWaves; ++q) wb[q][d] = 0;
     __syncthreads();
-    const long long wbase = (long long)blockIdx.x * kSortTile + w * (kSortRounds * 64);
+    const long long tbase = (long long)blockIdx.x * kSortTile;
+    const long long wbase = tbase + w * (kSortRounds * 64);
     int keys[kSortRounds];  // this lane's keys, kept for the ranking rounds
 #pragma unroll
     for (int r = 0; r < kSortRounds; ++r) {
@@ -165,13 +177,36 @@ k_sort_scatter(int M, const T *__restrict__ x, const T *__restrict__ y, const T 
         }
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < nd; d += 256) {
-        int run = counts[(long long)d * tiles + blockIdx.x];
+    // digit totals of the tile (thread t owns digits kPer t .. kPer t + kPer-1),
+    // their exclusive scan, and each wave's tile-local base per digit
+    int tot[kPer];
+    int sum = 0;
 #pragma unroll
-        for (int q = 0; q < kSortWaves; ++q) {
-            const int cq = wb[q][d];
-            wb[q][d] = run;
-            run += cq;
+    for (int k = 0; k < kPer; ++k) {
+        const int d = threadIdx.x * kPer + k;
+        tot[k] = 0;
+        if (d < nd)
+#pragma unroll
+            for (int q = 0; q < kSortWaves; ++q) tot[k] += wb[q][d];
+        sum += tot[k];
+    }
+    int2 ex;
+    block_exscan2(make_int2(sum, 0), ex);
+    int run = ex.x;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int d = threadIdx.x * kPer + k;
+        if (d < nd) {
+            dl[d] = run;
+            gd[d] = counts[(long long)d * tiles + blockIdx.x];
+            int b = run;
+#pragma unroll
+            for (int q = 0; q < kSortWaves; ++q) {
+                const int cq = wb[q][d];
+                wb[q][d] = b;
+                b += cq;
+            }
+            run += tot[k];
         }
     }
     __syncthreads();
@@ -188,21 +223,31 @@ k_sort_scatter(int M, const T *__restrict__ x, const T *__restrict__ y, const T 
             peers &= bit ? m : ~m;
         }
         const int rank = __popcll(peers & lt);
-        int dst = 0;
-        if (act) dst = wb[w][d] + rank;
+        int lp = 0;
+        if (act) lp = wb[w][d] + rank;
         if (act && rank == 0) wb[w][d] += __popcll(peers);
         if (act) {
-            SortRec<T, DIM> rec;
-            load_rec<T, DIM, FIRST>(j, x, y, z, rin, rec);
-            kout[dst] = keys[r];
-            if constexpr (LAST) {
-                idx[dst] = rec.i;
+            sk[lp] = keys[r];
+            sj[lp] = (int)j;
+        }
+    }
+    __syncthreads();
+    const int nt = (int)(M - tbase < kSortTile ? M - tbase : kSortTile);
+    for (int lp = threadIdx.x; lp < nt; lp += 256) {
+        const int key = sk[lp];
+        const int j = sj[lp];
+        const int d = (key >> shift) & (nd - 1);
+        const int dst = gd[d] + (lp - dl[d]);
+        SortRec<T, DIM> rec;
+        load_rec<T, DIM, FIRST>(j, x, y, z, rin, rec);
+        kout[dst] = key;
+        if constexpr (LAST) {
+            idx[dst] = rec.i;
 #pragma unroll
-                for (int a = 0; a < DIM; ++a)
-                    pos[a * stride + dst] = tap_pos(rescale_exact(rec.v[a], g.nf[a]), g.nf[a]);
-            } else {
-                rout[dst] = rec;
-            }
+            for (int a = 0; a < DIM; ++a)
+                pos[a * stride + dst] = tap_pos(rescale_exact(rec.v[a], g.nf[a]), g.nf[a]);
+        } else {
+            rout[dst] = rec;
         }
     }
 }
