@@ -232,22 +232,40 @@ k_sort_scatter(int M, const T *__restrict__ x, const T *__restrict__ y, const T 
         }
     }
     __syncthreads();
+    // write-out in batches of kBatch items per thread: all of a batch's
+    // record loads are issued before the first store, so their latencies
+    // overlap instead of adding up item by item
     const int nt = (int)(M - tbase < kSortTile ? M - tbase : kSortTile);
-    for (int lp = threadIdx.x; lp < nt; lp += 256) {
-        const int key = sk[lp];
-        const int j = sj[lp];
-        const int d = (key >> shift) & (nd - 1);
-        const int dst = gd[d] + (lp - dl[d]);
-        SortRec<T, DIM> rec;
-        load_rec<T, DIM, FIRST>(j, x, y, z, rin, rec);
-        kout[dst] = key;
-        if constexpr (LAST) {
-            idx[dst] = rec.i;
+    constexpr int kBatch = 8;
+    for (int b0 = 0; b0 < kSortTile; b0 += 256 * kBatch) {
+        SortRec<T, DIM> rec[kBatch];
+        int key[kBatch];
 #pragma unroll
-            for (int a = 0; a < DIM; ++a)
-                pos[a * stride + dst] = tap_pos(rescale_exact(rec.v[a], g.nf[a]), g.nf[a]);
-        } else {
-            rout[dst] = rec;
+        for (int k = 0; k < kBatch; ++k) {
+            const int lp = b0 + 256 * k + threadIdx.x;
+            key[k] = 0;
+            if (lp < nt) {
+                key[k] = sk[lp];
+                load_rec<T, DIM, FIRST>(sj[lp], x, y, z, rin, rec[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+            const int lp = b0 + 256 * k + threadIdx.x;
+            if (lp < nt) {
+                const int d = (key[k] >> shift) & (nd - 1);
+                const int dst = gd[d] + (lp - dl[d]);
+                kout[dst] = key[k];
+                if constexpr (LAST) {
+                    idx[dst] = rec[k].i;
+#pragma unroll
+                    for (int a = 0; a < DIM; ++a)
+                        pos[a * stride + dst] =
+                            tap_pos(rescale_exact(rec[k].v[a], g.nf[a]), g.nf[a]);
+                } else {
+                    rout[dst] = rec[k];
+                }
+            }
         }
     }
 }
