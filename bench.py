@@ -225,6 +225,22 @@ def algorithmic_bytes(w, nf, kind):
     return T * M * 2 * s + M * d * s + 4 * M + T * nftot * 2 * s
 
 
+def uses_tile_slots(w, info):
+    """Whether a type-1 SM plan stores single-subproblem tiles for the gather
+    pass (the library's rule: every axis with >= 3 bins, a halo of
+    ceil(ns/2) no wider than a bin, a tile no wider than the grid, and the
+    slots within 8 fine grids of memory); its spread stage is then the
+    spread kernel and the gather kernel."""
+    if w["type"] != 1 or info["method"] != 2:
+        return False
+    d, P = w["dim"], (info["ns"] + 1) // 2
+    nf, nb, bs = info["nf"][:d], info["numbins"][:d], info["binsize"][:d]
+    if any(n < 3 or P > b or b + 2 * P > f for n, b, f in zip(nb, bs, nf)):
+        return False
+    tile = int(np.prod([b + 2 * P for b in bs]))
+    return int(np.prod(nb)) * tile <= 8 * int(np.prod(nf))
+
+
 def run_plan(w, dev, dpts, din, opts, steps, warmup, maxbatch, accuracy_first=False):
     import torch
     from cufinufft_amd import cufinufft
@@ -589,6 +605,8 @@ def measure(w, wname, dev, dpts, din, sel, ref, a, accuracy_first=False):
     nbytes = algorithmic_bytes(w, best["info"]["nf"], "interp" if w["type"] == 2 else "spread")
     kname = ("k_interp_" if w["type"] == 2 else "k_spread_") + \
         {1: "gm", 2: "sm", 4: "blk"}[best["info"]["method"]]
+    slots = isinstance(best_opts, dict) and best_opts.get("tile_slots", False)
+    kstage = kname + (" + k_tile_gather" if slots and uses_tile_slots(w, best["info"]) else "")
     # template arguments <T, [Acc,] DIM, NS[, threads]>: match by name, type and NS
     ktmpl = (kname, "float" if dt == np.float32 else "double", w["dim"], best["info"]["ns"])
     out = dict(
@@ -599,7 +617,7 @@ def measure(w, wname, dev, dpts, din, sel, ref, a, accuracy_first=False):
                    method=plain["info"]["method"], ns=plain["info"]["ns"]),
         setpts=dict(pts_per_s=M / best["t_setpts"], ms=best["t_setpts"] * 1e3),
         stages_ms_per_exec={k: v[0] / max(a.steps, 1) for k, v in best["stages"].items()},
-        kernel=dict(name=kname, tmpl=ktmpl, ms=kern_ms, algorithmic_bytes=nbytes),
+        kernel=dict(name=kstage, tmpl=ktmpl, ms=kern_ms, algorithmic_bytes=nbytes),
     )
     del plain, best
     return out

@@ -94,6 +94,7 @@ _make_planf_ext.argtypes = [c_int, c_int, c_int_p, c_int, c_int, c_float, c_int,
 _make_planf_ext.restype = c_int
 
 ACCURACY_FIRST = 1
+TILE_SLOTS = 2
 
 _set_pts = lib.cufinufft_setpts
 _set_pts.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_int, c_double_p,

@@ -212,6 +212,13 @@ def tune(nufft_type, modes, pts, eps, dtype=np.float32, n_trans=1, maxbatchsize=
             t = trial(dict(best, gpu_kerevalmeth=1), long_reps)
             if t is not None and t < best_t:
                 best, best_t = dict(best, gpu_kerevalmeth=1), t
+        # type-1 SM: stored tiles and a gather pass instead of atomic flushes
+        # (pays where most bins hold one subproblem)
+        if nufft_type == 1 and best.get("gpu_method", 2) == 2:
+            t = trial(dict(best, tile_slots=True), long_reps)
+            if t is not None and t < best_t:
+                best, best_t = dict(best, tile_slots=True), t
+        if upsampfac == 2.0:
             # sigma = 1.25 widens the kernel (ns 9 instead of 6 at tol 1e-5,
             # 15 instead of 10 at 1e-9), so the winner's bins may not fit the
             # LDS with it: halve the largest bin side until a plan is made

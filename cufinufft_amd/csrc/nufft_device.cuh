@@ -968,7 +968,8 @@ __device__ __forceinline__ void spread_sm_job(const SubprobJob &J, int M, const 
                                               const cpx<T> *__restrict__ c,
                                               cpx<T> *__restrict__ fw, const BinGeom &g,
                                               const KerParams<T> &kp, long long nftot,
-                                              unsigned char *smem)
+                                              unsigned char *smem,
+                                              cpx<T> *__restrict__ slot = nullptr)
 {
     constexpr bool DRAIN = sizeof(Acc) == 4;
     constexpr int NK = sizeof(Acc) == 4 ? 64 : 32;
@@ -1095,7 +1096,28 @@ __device__ __forceinline__ void spread_sm_job(const SubprobJob &J, int M, const 
     __syncthreads();
     const double back = ldexp(eb, -sh);
     cpx<T> *f = fw + (long long)J.t * nftot;
-    if constexpr (DRAIN) {
+    if (slot) {
+        // the bin's only subproblem: store the whole tile, in tile order
+        // (coalesced plain stores; the gather pass adds it to the grid)
+        cpx<T> *sl = slot + ((long long)J.t * g.nb[0] * g.nb[1] * g.nb[2] + J.b) * n;
+        const float bf = (float)back;
+        const bool drained = DRAIN && nch > 1;
+        for (int p = tid; p < n; p += BT) {
+            cpx<T> v;
+            if constexpr (DRAIN) {
+                v.x = (float)sre[p] * bf;
+                v.y = (float)sim[p] * bf;
+                if (drained) {
+                    v.x += dre[p];
+                    v.y += dimg[p];
+                }
+            } else {
+                v.x = (T)((double)sre[p] * back);
+                v.y = (T)((double)sim[p] * back);
+            }
+            sl[p] = v;
+        }
+    } else if constexpr (DRAIN) {
         const float bf = (float)back;
         const bool drained = nch > 1;
         for_tile_cells(tl, g, tid, BT, [&](int p, long long off) {
@@ -1130,17 +1152,102 @@ template <typename T, typename Acc, int DIM, int NS, int BT>
 __global__ void __launch_bounds__(BT)
 k_spread_sm(int M, SortedPts sp, const cpx<T> *__restrict__ c, cpx<T> *__restrict__ fw,
             BinGeom g, KerParams<T> kp, SubprobArgs sa, int ntr, long long nftot,
-            unsigned long long *jq)
+            unsigned long long *jq, cpx<T> *__restrict__ slot)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int total = sa.counters[1];
     jobq_run(
         jq, (long long)total * ntr,
         [&](long long job) {
-            spread_sm_job<T, Acc, DIM, NS, BT>(subprob_job(job, total, sa), M, sp, c, fw, g, kp,
-                                               nftot, smem);
+            const SubprobJob J = subprob_job(job, total, sa);
+            // a bin of one subproblem stores its tile to its slot (when the
+            // plan has them); bins of several keep adding into the grid
+            cpx<T> *sl = slot && sa.substart[J.b + 1] - sa.substart[J.b] == 1 ? slot : nullptr;
+            spread_sm_job<T, Acc, DIM, NS, BT>(J, M, sp, c, fw, g, kp, nftot, smem, sl);
         },
         [&](long long job) { return subprob_pos(job, total, M, sa); }, (long long)M * ntr);
+}
+
+// Tiles covering grid index gc along one axis of nb bins of bs cells (grid
+// nf, halo P, tile bs + 2P wide, tile of bin u starting at u bs - P): bins
+// up to two away from the cell's own (two, because the halo of the bin
+// before a partial last bin narrower than P reaches past it, across the
+// period), a wrapped bin's tile one period away.  Writes up to five (bin,
+// tile coordinate) pairs, returns how many.  Complete (every tile cell a
+// point can reach that maps to gc is listed, none twice) when nb >= 3,
+// P <= bs and the tile is no wider than the grid; the far end of a partial
+// last bin's tile, which no point reaches, may be left out.
+CFA_HD int tile_cover_axis(int gc, int nb, int bs, int nf, int P, int *cb, int *cq)
+{
+    const int tw = bs + 2 * P;
+    const int b = gc / bs < nb - 1 ? gc / bs : nb - 1;
+    int n = 0;
+    for (int k = -2; k <= 2; ++k) {
+        int u = b + k, shift = 0;
+        if (u < 0) {
+            u += nb;
+            shift = -nf;
+        } else if (u >= nb) {
+            u -= nb;
+            shift = nf;
+        }
+        const int q = gc - (u * bs - P + shift);
+        if (q >= 0 && q < tw) {
+            cb[n] = u;
+            cq[n] = q;
+            ++n;
+        }
+    }
+    return n;
+}
+
+// Tile gather after an SM spread with slots: every fine-grid cell adds the
+// stored tiles of the single-subproblem bins that cover it (tile_cover_axis
+// along each axis), in a fixed order.  Cells of bins with several
+// subproblems already hold those bins' atomic flushes.  One workgroup per
+// grid row (y, z) and transform: the row's covers along y and z are found
+// once, and the threads walk x, so a wave reads runs of one tile row per
+// covering bin and the grid row in order.
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256)
+k_tile_gather(cpx<T> *__restrict__ fw, const cpx<T> *__restrict__ slot, BinGeom g, int P,
+              const int *__restrict__ substart, long long nftot)
+{
+    const long long nbins = (long long)g.nb[0] * g.nb[1] * g.nb[2];
+    const int tw0 = g.bs[0] + 2 * P;
+    const int tw1 = DIM > 1 ? g.bs[1] + 2 * P : 1;
+    const int tw2 = DIM > 2 ? g.bs[2] + 2 * P : 1;
+    const long long tcells = (long long)tw0 * tw1 * tw2;
+    const int rows = (DIM > 1 ? g.nf[1] : 1) * (DIM > 2 ? g.nf[2] : 1);
+    const int t = blockIdx.x / rows;
+    const int row = blockIdx.x - t * rows;
+    const int gy = DIM > 1 ? row % g.nf[1] : 0;
+    const int gz = DIM > 2 ? row / g.nf[1] : 0;
+    int yb[5], yq[5], zb[5], zq[5];
+    int ny = 1, nz = 1;
+    yb[0] = yq[0] = zb[0] = zq[0] = 0;
+    if (DIM > 1) ny = tile_cover_axis(gy, g.nb[1], g.bs[1], g.nf[1], P, yb, yq);
+    if (DIM > 2) nz = tile_cover_axis(gz, g.nb[2], g.bs[2], g.nf[2], P, zb, zq);
+    cpx<T> *f = fw + (long long)t * nftot + (long long)row * g.nf[0];
+    const cpx<T> *sl = slot + (long long)t * nbins * tcells;
+    for (int gx = threadIdx.x; gx < g.nf[0]; gx += blockDim.x) {
+        int xb[5], xq[5];
+        const int nx = tile_cover_axis(gx, g.nb[0], g.bs[0], g.nf[0], P, xb, xq);
+        cpx<T> acc = f[gx];
+        for (int iz = 0; iz < nz; ++iz)
+            for (int iy = 0; iy < ny; ++iy) {
+                const long long brow = (long long)g.nb[0] * (yb[iy] + (long long)g.nb[1] * zb[iz]);
+                const long long qrow = (long long)tw0 * (yq[iy] + (long long)tw1 * zq[iz]);
+                for (int ix = 0; ix < nx; ++ix) {
+                    const long long bin = brow + xb[ix];
+                    if (substart[bin + 1] - substart[bin] != 1) continue;
+                    const cpx<T> v = sl[bin * tcells + qrow + xq[ix]];
+                    acc.x += v.x;
+                    acc.y += v.y;
+                }
+            }
+        f[gx] = acc;
+    }
 }
 
 // One member of a plan group (many independent plans of one type, dimension,

@@ -35,6 +35,10 @@ struct SortBufs {
     double *pos = nullptr;      // [dim][pos_stride] sorted rescaled positions
     long long pos_stride = 0;   // capacity in points of each pos plane
     int *counters = nullptr;    // [2]: points, subproblems
+    // SM spread (type 1): the padded tile of every bin that one subproblem
+    // covers, stored whole instead of added to the fine grid
+    // ([transform][bin][tile cell] complex; null when not allocated)
+    void *slot = nullptr;
     unsigned long long *jobq = nullptr;  // [kJobqWords] SM job queue words, zero between launches
     void *scan_part = nullptr;  // int2[ceil(nbins / chunk)]
     // method 4: point copies counted per block

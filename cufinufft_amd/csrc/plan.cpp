@@ -501,6 +501,8 @@ struct Plan {
         dfree(sb.kstart);
         if (sb.kscan_part) hipFree(sb.kscan_part);
         sb.kscan_part = nullptr;
+        if (sb.slot) hipFree(sb.slot);
+        sb.slot = nullptr;
     }
 
     int make_fft(hipfftHandle *h, int batch)
@@ -610,7 +612,7 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
 {
     if (type_ != 1 && type_ != 2) return 1;
     if (dim_ < 1 || dim_ > 3) return 1;
-    if (flags & ~CUFINUFFT_ACCURACY_FIRST) return 1;
+    if (flags & ~(CUFINUFFT_ACCURACY_FIRST | CUFINUFFT_TILE_SLOTS)) return 1;
     // Accuracy-first: the reference's width rule ns = ceil(-log10(tol/10))
     // leaves the transform's relative l2 error at 0.6-1.1, 1-2 and 1.5-3.2
     // times tol in 1, 2 and 3 dimensions (the same algorithm in FP64 against
@@ -773,6 +775,32 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
         return kErrHip;
     const long long nparts = (nbins + kScanChunk - 1) / kScanChunk;
     if (hipMalloc(&sb.scan_part, sizeof(int2) * (size_t)nparts) != hipSuccess) return kErrHip;
+    // Tile slots of the SM spread (CUFINUFFT_TILE_SLOTS): a bin with one
+    // subproblem stores its whole padded tile with plain stores and a gather
+    // pass adds the tiles covering each cell, instead of flushing the tile
+    // with two global float atomics per cell (on uniform 3D points, 0.4 per
+    // cell, those atomics were half of the spread; the gather's pass over the
+    // whole grid costs more than it saves where most bins are dense or
+    // empty, hence a flag).  The gather reads the tiles of a cell's own bin and its two
+    // neighbours per axis, so every axis needs three bins or more, a halo no
+    // wider than a bin and a tile no wider than the grid; and the slots are
+    // kept within 8 fine grids of memory.
+    if (type == 1 && method == 2 && (flags & CUFINUFFT_TILE_SLOTS)) {
+        const int P = (sp.ns + 1) / 2;
+        bool ok = true;
+        size_t cells = 1;
+        for (int d = 0; d < dim; ++d) {
+            const int tw = geom.bs[d] + 2 * P;
+            ok = ok && geom.nb[d] >= 3 && P <= geom.bs[d] && tw <= nf[d];
+            cells *= (size_t)tw;
+        }
+        const size_t bytes = (size_t)nbins * cells * maxbatch * sizeof(cpx<T>);
+        const size_t grid = (size_t)nftot * maxbatch * sizeof(cpx<T>);
+        if (ok && bytes <= 8 * grid && hipMalloc(&sb.slot, bytes) != hipSuccess) {
+            sb.slot = nullptr;
+            (void)hipGetLastError();  // no slots: the spread flushes with atomics
+        }
+    }
     if (opts.gpu_sort == 2 &&
         nbins * (long long)geom.bs[0] * geom.bs[1] * geom.bs[2] > 2147483647LL)
         return kErrNdataNotValid;  // (bin, cell) keys are int

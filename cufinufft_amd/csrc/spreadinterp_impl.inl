@@ -81,8 +81,9 @@ int spread_ns(int method, int M, const SortBufs &sb, const cpx<T> *c, cpx<T> *fw
                                                            &grid, BT);                        \
         if (e) return e;                                                                      \
         k_spread_sm<T, ACC, DIM, NS, BT><<<grid, BT, cfg.lds, L.stream>>>(                    \
-            M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq);                        \
+            M, sorted_pts(sb), c, fw, g, kp, sa, ntr, nftot, sb.jobq, slot);                  \
     } while (0)
+        cpx<T> *slot = static_cast<cpx<T> *>(sb.slot);
         if constexpr (sizeof(T) == 4) {
             if (cfg.acc_bytes == 4)
                 CFA_SM_SPREAD(int, 512);
@@ -97,6 +98,13 @@ int spread_ns(int method, int M, const SortBufs &sb, const cpx<T> *c, cpx<T> *fw
                 CFA_SM_SPREAD(long long, 256);
         }
 #undef CFA_SM_SPREAD
+        if (slot) {
+            // add the stored tiles of the single-subproblem bins: one
+            // workgroup per grid row and transform
+            const long long rows = nftot / g.nf[0] * ntr;
+            k_tile_gather<T, DIM><<<(unsigned)rows, 256, 0, L.stream>>>(
+                fw, slot, g, (NS + 1) / 2, sb.substart, nftot);
+        }
     }
     return hipGetLastError() == hipSuccess ? 0 : kErrHip;
 }

@@ -256,6 +256,50 @@ static int run(const std::vector<int> &nf, long long M, const int *bs, int maxsu
             });
     }
     std::printf("spread: %lld jobs, %lld tile cells flushed\n", subs, flushed);
+
+    // ---- tile slots and their gather (plans where every axis has >= 3 bins,
+    // a halo no wider than a bin and a tile no wider than the grid): along
+    // each axis, the (bin, tile coordinate) pairs the gather lists for a grid
+    // index must map to it, and must include every pair that maps to it and
+    // that a point of the bin can reach (within P of the bin's cells)
+    {
+        const int Ph = (NS + 1) / 2;
+        bool slots = true;
+        for (int d = 0; d < DIM; ++d)
+            slots = slots && g.nb[d] >= 3 && Ph <= g.bs[d] && g.bs[d] + 2 * Ph <= g.nf[d];
+        long long pairs = 0;
+        for (int d = 0; slots && d < DIM; ++d) {
+            const int nb = g.nb[d], bs = g.bs[d], nfd = g.nf[d], tw = bs + 2 * Ph;
+            for (int gc = 0; gc < nfd; ++gc) {
+                int cb[5], cq[5];
+                const int n = tile_cover_axis(gc, nb, bs, nfd, Ph, cb, cq);
+                CHECK(n >= 1 && n <= 5, "axis %d cell %d: %d covering tiles", d, gc, n);
+                for (int k = 0; k < n; ++k) {
+                    const int c = cb[k] * bs - Ph + cq[k];
+                    CHECK(cb[k] >= 0 && cb[k] < nb && cq[k] >= 0 && cq[k] < tw &&
+                              ((c % nfd) + nfd) % nfd == gc,
+                          "axis %d cell %d: tile (%d, %d) maps elsewhere", d, gc, cb[k], cq[k]);
+                    for (int k2 = 0; k2 < k; ++k2)
+                        CHECK(cb[k2] != cb[k] || cq[k2] != cq[k], "axis %d cell %d: pair twice", d,
+                              gc);
+                }
+                for (int u = 0; u < nb; ++u) {
+                    const int end = std::min((u + 1) * bs, nfd);
+                    for (int q = 0; q < tw; ++q) {
+                        const int c = u * bs - Ph + q;
+                        if (((c % nfd) + nfd) % nfd != gc || c >= end + Ph) continue;
+                        bool listed = false;
+                        for (int k = 0; k < n; ++k) listed = listed || (cb[k] == u && cq[k] == q);
+                        CHECK(listed, "axis %d cell %d: reachable tile (%d, %d) not gathered", d,
+                              gc, u, q);
+                        ++pairs;
+                    }
+                }
+            }
+        }
+        std::printf("tile slots: %s, %lld reachable (bin, cell) pairs checked\n",
+                    slots ? "used" : "not used", pairs);
+    }
     std::printf("%s: %d failed checks\n", g_fail ? "FAILED" : "OK", g_fail);
     return g_fail ? 1 : 0;
 }

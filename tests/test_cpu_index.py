@@ -34,6 +34,10 @@ CASES = [
     "d 1 8192 1 1 9 100000 1024 1 1 1024 1e-8 mixed",
     "f 3 30 30 20 9 100000 8 8 8 1024 1e-5 mixed",      # sigma 1.25 widths
     "d 3 30 30 30 15 50000 4 4 4 1024 1e-9 mixed",
+    # tile slots with partial last bins (gather covering rule)
+    "f 3 33 40 26 6 50000 8 8 8 256 1e-5 mixed",
+    "d 2 70 45 1 10 30000 16 16 1 2048 1e-9 mixed",
+    "f 1 300 1 1 7 8000 32 1 1 2048 1e-6 mixed",
     # keys wide enough for two and three counting passes
     "f 3 256 256 256 6 200000 4 4 4 1024 1e-5 mixed",
     "f 3 512 512 512 6 100000 2 2 2 1024 1e-5 uniform",

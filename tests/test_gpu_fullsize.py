@@ -12,7 +12,8 @@ k . x_j in FP64, cos / sin in FP64, streamed over point chunks).
 
 Bars: relative l2 over the sampled modes <= 2 tol (the transform's stated
 accuracy with the 2x slack), for the reference's default options and for a
-tuned layout (16x8x8 bins, 4096-point subproblems, polynomial kernel); and
+tuned layout (16x8x8 bins, 4096-point subproblems, polynomial kernel, tile
+slots with the gather pass); and
 the two option sets agree to tol / 5 on the whole mode array (same
 algorithm, different bins and accumulation order)."""
 import numpy as np
@@ -83,7 +84,7 @@ def test_type1_fp32_fullsize(torch_dev):
     for name, opts in (("plain", {}),
                        ("tuned", dict(gpu_method=2, gpu_binsizex=16, gpu_binsizey=8,
                                       gpu_binsizez=8, gpu_maxsubprobsize=4096,
-                                      gpu_kerevalmeth=1))):
+                                      gpu_kerevalmeth=1, tile_slots=True))):
         plan = cufinufft(1, N[::-1], eps=TOL, dtype=np.float32, **opts)
         plan.set_pts(*xyz[::-1])
         fk = torch.empty(N[::-1], dtype=torch.complex64, device=torch_dev)

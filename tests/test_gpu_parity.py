@@ -204,6 +204,54 @@ def test_spread_only_chunked_32bit(torch_dev, case):
     assert scale_err(fw, ref) < 3e-7 + 4 * err / np.sqrt(np.count_nonzero(ref))
 
 
+SLOT_CASES = [
+    # dim, dtype, modes (= the grid), M, tol, bins, maxsub, ntransf
+    (3, np.float32, (33, 40, 26), 60000, 1e-5, (8, 8, 8), 256, 3),
+    (2, np.float64, (70, 45), 20000, 1e-9, (16, 16), 2048, 2),
+    (1, np.float32, (300,), 8000, 1e-6, (32,), 2048, 1),
+]
+
+
+@pytest.mark.parametrize("case", SLOT_CASES, ids=lambda c: f"{c[0]}d-{np.dtype(c[1]).name}")
+def test_spread_only_tile_slots(torch_dev, case):
+    """With the tile-slot flag the SM spread stores the tile of every
+    single-subproblem bin whole and a gather pass adds the tiles covering
+    each cell (when every axis has 3
+    bins or more, a halo no wider than a bin and a tile no wider than the
+    grid).  Grids whose last bin is partial (33 = 4x8 + 1, 26 = 3x8 + 2,
+    45 = 2x16 + 13, 300 = 9x32 + 12), half the points uniform (single-
+    subproblem bins, through the slots) and half in one tight cluster (bins of
+    many subproblems, flushed with atomics into the same cells), several
+    transforms in batches of two: against the oracle's FP64 sum.  Bars as for
+    the chunked spread: FP32 within tol/5 (the cluster cells take ~100 FP32
+    atomic flushes each) or the 8e-7 FP32 kernel-value level of the uniform
+    spread-only bar, whichever is larger; FP64 within 1e-12; and the
+    least-squares scale within the kernel-value level plus the noise's
+    share."""
+    dim, dtype, nm, M, tol, bins, maxsub, ntr = case
+    _flush(f"spread-only tile slots {case}")
+    u = G.uniform_pts(M // 2, dim, dtype, seed=12)
+    cl = G.clustered_pts(M - M // 2, dim, dtype, seed=12, ncl=1, sigma=0.15)
+    pts = [np.concatenate([a, b]) for a, b in zip(u, cl)]
+    c = G.strengths(M, dtype, seed=12, ntr=ntr)
+    b3 = tuple(bins) + (1,) * (3 - dim)
+    fw, plan = G.run_type1(torch_dev, pts, c, nm, tol, dtype, ntr=ntr, maxbatchsize=2,
+                           tile_slots=True, gpu_method=2, gpu_spreadinterponly=1,
+                           gpu_maxsubprobsize=maxsub,
+                           gpu_binsizex=b3[0], gpu_binsizey=b3[1], gpu_binsizez=b3[2])
+    nsub = np.diff(plan.sort_arrays()["substart"])
+    assert np.any(nsub == 1) and np.any(nsub > 1)
+    sp = O.setup_spreader(tol, dtype == np.float32)
+    single = dtype == np.float32
+    for t in range(ntr):
+        ref = O.spread(pts, c[t * M:(t + 1) * M], list(nm), sp)
+        got = fw[t] if ntr > 1 else fw
+        err = rel_l2(got, ref)
+        assert err < (max(tol / 5, 8e-7) if single else 1e-12), (t, err)
+        assert scale_err(got, ref) < ((3e-7 + 4 * err / np.sqrt(np.count_nonzero(ref)))
+                                      if single else 1e-12)
+
+
 @pytest.mark.parametrize("method", [1, 2])
 @pytest.mark.parametrize("case", SI_CASES, ids=lambda c: f"{c[0]}d-{np.dtype(c[1]).name}")
 def test_interp_only(torch_dev, case, method):
