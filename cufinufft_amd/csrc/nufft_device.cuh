@@ -1204,12 +1204,12 @@ CFA_HD int tile_cover_axis(int gc, int nb, int bs, int nf, int P, int *cb, int *
 // Tile gather after an SM spread with slots: every fine-grid cell adds the
 // stored tiles of the single-subproblem bins that cover it (tile_cover_axis
 // along each axis), in a fixed order.  Cells of bins with several
-// subproblems already hold those bins' atomic flushes.  One workgroup per
-// grid row (y, z) and transform: the row's covers along y and z are found
-// once, and the threads walk x, so a wave reads runs of one tile row per
-// covering bin and the grid row in order.
+// subproblems already hold those bins' atomic flushes.  One one-wave
+// workgroup per grid row (y, z) and transform: the row's covers along y and
+// z are found once, and the lanes walk x, so the wave reads runs of one tile
+// row per covering bin and the grid row in order.
 template <typename T, int DIM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64)
 k_tile_gather(cpx<T> *__restrict__ fw, const cpx<T> *__restrict__ slot, BinGeom g, int P,
               const int *__restrict__ substart, long long nftot)
 {
@@ -1228,22 +1228,49 @@ k_tile_gather(cpx<T> *__restrict__ fw, const cpx<T> *__restrict__ slot, BinGeom 
     yb[0] = yq[0] = zb[0] = zq[0] = 0;
     if (DIM > 1) ny = tile_cover_axis(gy, g.nb[1], g.bs[1], g.nf[1], P, yb, yq);
     if (DIM > 2) nz = tile_cover_axis(gz, g.nb[2], g.bs[2], g.nf[2], P, zb, zq);
+    // which candidate bins of the row hold one subproblem: a byte per
+    // (y/z cover, x bin) in LDS when they fit, read back without a memory
+    // round trip, so each cell's slot loads (all x covers of a y/z cover,
+    // under a lane mask) go out together instead of one chain per tile
+    __shared__ unsigned char sflag[4096];
+    const int nbx = g.nb[0];
+    const bool lflag = ny * nz * nbx <= 4096;
+    if (lflag) {
+        for (int i = threadIdx.x; i < ny * nz * nbx; i += blockDim.x) {
+            const int combo = i / nbx, bx = i - combo * nbx;
+            const long long bin =
+                bx + (long long)nbx * (yb[combo % ny] + (long long)g.nb[1] * zb[combo / ny]);
+            sflag[i] = substart[bin + 1] - substart[bin] == 1;
+        }
+        __syncthreads();
+    }
     cpx<T> *f = fw + (long long)t * nftot + (long long)row * g.nf[0];
     const cpx<T> *sl = slot + (long long)t * nbins * tcells;
     for (int gx = threadIdx.x; gx < g.nf[0]; gx += blockDim.x) {
         int xb[5], xq[5];
-        const int nx = tile_cover_axis(gx, g.nb[0], g.bs[0], g.nf[0], P, xb, xq);
+        const int nx = tile_cover_axis(gx, nbx, g.bs[0], g.nf[0], P, xb, xq);
         cpx<T> acc = f[gx];
         for (int iz = 0; iz < nz; ++iz)
             for (int iy = 0; iy < ny; ++iy) {
-                const long long brow = (long long)g.nb[0] * (yb[iy] + (long long)g.nb[1] * zb[iz]);
+                const long long brow = (long long)nbx * (yb[iy] + (long long)g.nb[1] * zb[iz]);
                 const long long qrow = (long long)tw0 * (yq[iy] + (long long)tw1 * zq[iz]);
-                for (int ix = 0; ix < nx; ++ix) {
-                    const long long bin = brow + xb[ix];
-                    if (substart[bin + 1] - substart[bin] != 1) continue;
-                    const cpx<T> v = sl[bin * tcells + qrow + xq[ix]];
-                    acc.x += v.x;
-                    acc.y += v.y;
+                const int frow = (iz * ny + iy) * nbx;
+                cpx<T> v[5];
+#pragma unroll
+                for (int ix = 0; ix < 5; ++ix) {
+                    v[ix].x = T(0);
+                    v[ix].y = T(0);
+                    if (ix < nx) {
+                        const long long bin = brow + xb[ix];
+                        const bool single = lflag ? sflag[frow + xb[ix]] != 0
+                                                  : substart[bin + 1] - substart[bin] == 1;
+                        if (single) v[ix] = sl[bin * tcells + qrow + xq[ix]];
+                    }
+                }
+#pragma unroll
+                for (int ix = 0; ix < 5; ++ix) {
+                    acc.x += v[ix].x;
+                    acc.y += v[ix].y;
                 }
             }
         f[gx] = acc;
