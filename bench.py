@@ -693,7 +693,10 @@ def main():
         if not a.opts:
             log("[bench] --kernel-only needs --opts")
             return 2
-        r = run_plan(w, dev, dpts, din, json.loads(a.opts), a.steps, a.warmup, maxbatch)
+        # FP64 workloads below tol 1e-7 are planned accuracy-first, as in the
+        # measured line
+        r = run_plan(w, dev, dpts, din, json.loads(a.opts), a.steps, a.warmup, maxbatch,
+                     accuracy_first=w["dtype"] == "float64" and w["tol"] < 1e-7)
         log(f"[bench] kernel-only: {r['t_exec'] * 1e3:.3f} ms/exec, setpts "
             f"{r['t_setpts'] * 1e3:.3f} ms, stages {r['stages']}")
         return 0
