@@ -352,7 +352,8 @@ def cpu_baseline(w, pts, host_in, budget_pts):
 def bench_group(a, dev):
     """64 independent 2D type-1 plans (N per axis log-uniform in [64, 512],
     M log-uniform in [1e3, 1e6], FP32, tol 1e-6): sequential plain plans vs
-    one plan group (one persistent spread launch) eager and hipGraph-replayed."""
+    one plan group (one persistent spread launch) eager and hipGraph-replayed,
+    with the group's bins and subproblem size picked by the autotuner."""
     import torch
     from cufinufft_amd import cufinufft
     from cufinufft_amd.cufinufft import cufinufft_group
@@ -396,7 +397,16 @@ def bench_group(a, dev):
     t_plain = timeit(lambda: [p.execute(d[4], o) for p, d, o in zip(plans, data, outs)],
                      a.steps, a.warmup)
     del plans, outs
-    opts = dict(gpu_method=2, gpu_binsizex=32, gpu_binsizey=32, gpu_maxsubprobsize=1024)
+    from cufinufft_amd import autotune
+
+    def build(o):
+        ps, os_ = make(o)
+        g = cufinufft_group(ps, [d[4] for d in data], os_)
+        return lambda: g.execute(True)
+
+    log("[bench] tuning the group's bins and subproblem size (cufinufft_amd.autotune)")
+    opts, _ = autotune.tune_group(build, autotune.group_candidates(2), log=log)
+    log(f"[bench] group choice {opts}")
     plans, outs = make(opts)
     grp = cufinufft_group(plans, [d[4] for d in data], outs)
     t_eager = timeit(lambda: grp.execute(False), a.steps, a.warmup)

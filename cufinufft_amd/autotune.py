@@ -233,3 +233,53 @@ def tune(nufft_type, modes, pts, eps, dtype=np.float32, n_trans=1, maxbatchsize=
     if use_cache:
         _save_disk()
     return dict(best), table
+
+
+def group_candidates(dim):
+    """Option sets to try for a plan group (one SM launch over every member,
+    so one method -- 2 -- and one set of bins and subproblem size, whose
+    largest tile sizes the launch's LDS)."""
+    if dim == 3:
+        bins = [(16, 8, 8), (8, 8, 8), (16, 16, 4)]
+    elif dim == 2:
+        bins = [(32, 32, 1), (16, 16, 1), (64, 32, 1), (32, 16, 1), (64, 64, 1)]
+    else:
+        bins = [(1024, 1, 1), (512, 1, 1)]
+    return [dict(gpu_method=2, gpu_binsizex=b[0], gpu_binsizey=b[1], gpu_binsizez=b[2],
+                 gpu_maxsubprobsize=msub) for b in bins for msub in (512, 1024, 4096)]
+
+
+def tune_group(build, cands, reps=5, log=None):
+    """Time a plan group under each candidate option set.
+
+    build(opts) makes the members with opts (points set), forms the group and
+    returns a callable that executes it once (or raises RuntimeError when
+    the options cannot be planned).  Returns (best_opts, [(opts, ms) ...]).
+    """
+    import torch
+    table = []
+    best, best_t = None, float("inf")
+    for opts in cands:
+        try:
+            run = build(opts)
+            run()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                run()
+            torch.cuda.synchronize()
+            t = (time.perf_counter() - t0) / reps * 1e3
+        except RuntimeError as e:
+            if log:
+                log(f"  skip {opts}: {e}")
+            continue
+        finally:
+            run = None
+        table.append((opts, t))
+        if log:
+            log(f"  group {opts}: {t:.3f} ms")
+        if t < best_t:
+            best, best_t = opts, t
+    if best is None:
+        raise RuntimeError("no candidate option set could be planned for the group")
+    return dict(best), table

@@ -52,3 +52,47 @@ def test_tune_then_transform(torch_dev, nufft_type):
         err = rel_l2(cc[pidx], direct)
     print(f"   tuned {best}: gpu err {err:.3e} oracle err {oerr:.3e}", flush=True)
     assert err <= max(2.0 * tol, 1.25 * oerr)
+
+
+def test_tune_group(torch_dev):
+    """Group tuning: every candidate option set is planned for all members
+    and the group timed as one launch; the chosen set is one of those timed,
+    and the group run with it matches its members' own executes."""
+    import torch
+    from cufinufft_amd import autotune, cufinufft
+    from cufinufft_amd.cufinufft import cufinufft_group
+    print("autotune plan group", flush=True)
+    tol = 1e-5
+    shapes = [((48, 40), 6000), ((64, 64), 15000), ((40, 56), 3000)]
+    data = []
+    for i, (nm, M) in enumerate(shapes):
+        pts = G.uniform_pts(M, 2, np.float32, seed=40 + i)
+        c = G.to_dev(G.strengths(M, np.float32, seed=40 + i), torch_dev)
+        data.append((nm, [G.to_dev(p, torch_dev) for p in pts[::-1]], c))
+
+    def make(opts):
+        plans, outs = [], []
+        for nm, dpts, _ in data:
+            p = cufinufft(1, nm[::-1], eps=tol, dtype=np.float32, **opts)
+            p.set_pts(*dpts)
+            plans.append(p)
+            outs.append(torch.zeros(nm[::-1], dtype=torch.complex64, device=torch_dev))
+        return plans, outs
+
+    def build(opts):
+        plans, outs = make(opts)
+        g = cufinufft_group(plans, [d[2] for d in data], outs)
+        return lambda: g.execute(True)
+
+    cands = autotune.group_candidates(2)[:4]
+    best, table = autotune.tune_group(build, cands, reps=2)
+    assert best in [o for o, _ in table] and len(table) == len(cands)
+    plans, outs = make(best)
+    grp = cufinufft_group(plans, [d[2] for d in data], outs)
+    grp.execute(True)
+    torch.cuda.synchronize()
+    for p, (nm, _, c), o in zip(plans, data, outs):
+        ref = torch.zeros_like(o)
+        p.execute(c, ref)
+        torch.cuda.synchronize()
+        assert rel_l2(o.cpu().numpy(), ref.cpu().numpy()) < 1e-6
