@@ -57,7 +57,8 @@ def build(jobs=None, verbose=True):
                 print(f"  compiled {os.path.basename(obj)} in {dt:.1f}s", flush=True)
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp,
-           "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"]
+           "-L/opt/rocm/lib", "-lhipfft", "-lrocprofiler-sdk-roctx",
+           "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")

@@ -8,6 +8,7 @@
 // is asynchronous on the plan's stream.
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <cmath>
@@ -38,6 +39,20 @@ constexpr int kStageSpreadInterp = 2;   // spread (type 1) or interp (type 2)
 constexpr int kStageFft = 3;            // hipFFT
 constexpr int kStageDeconv = 4;         // deconvolve (type 1) or amplify (type 2)
 constexpr int kNumStages = 8;
+// roctx range names of the stages (shown by rocprofv3 --marker-trace).
+constexpr const char *kStageName[5] = {"cufinufft sort", "cufinufft grid zero",
+                                       "cufinufft spread/interp", "cufinufft fft",
+                                       "cufinufft deconvolve/amplify"};
+
+// roctx range over one API call (the reference's NVTX ranges,
+// include/profile.h:14-15, src/profile.cu:10-31).  A push and pop per call;
+// without a tool attached the library's dispatch is a no-op.
+struct ApiRange {
+    explicit ApiRange(const char *name) { roctxRangePushA(name); }
+    ~ApiRange() { roctxRangePop(); }
+    ApiRange(const ApiRange &) = delete;
+    ApiRange &operator=(const ApiRange &) = delete;
+};
 
 // ---------------------------------------------------------------------------
 // Spreader parameters, contrib/spreadinterp.cpp:6-67, evaluated in the plan
@@ -462,6 +477,24 @@ struct Plan {
         spans.push_back({stage, ev_open, b});
         ev_open = nullptr;
     }
+    // One stage: a roctx range and, while timing is on, an event span, both
+    // closed when the scope ends (every return path included).
+    struct Stage {
+        Plan *p;
+        int id;
+        Stage(Plan *p_, int id_) : p(p_), id(id_)
+        {
+            roctxRangePushA(kStageName[id]);
+            p->tbegin();
+        }
+        ~Stage()
+        {
+            p->tend(id);
+            roctxRangePop();
+        }
+        Stage(const Stage &) = delete;
+        Stage &operator=(const Stage &) = delete;
+    };
 
     ~Plan() { release(); }
 
@@ -823,7 +856,7 @@ int Plan<T>::setpts_blocks()
     const int maxsub = opts.gpu_maxsubprobsize;
     const int obv[3] = {geom.bs[0], geom.bs[1], geom.bs[2]};
     if ((long long)M * blk_max_copies(dim, sp.ns, obv) > 2147483647LL) return kErrNdataNotValid;
-    tbegin();
+    Stage st(this, kStageSort);
     int e = run_blk_count<T>(dim, geom, sp.ns, M, kx, ky, kz, sb, L);
     if (e) return e;
     int ncopies = 0;
@@ -852,7 +885,6 @@ int Plan<T>::setpts_blocks()
         cap_sub = sub_bound;
     }
     e = run_blk_scatter<T>(dim, geom, sp.ns, M, ncopies, kx, ky, kz, maxsub, sb, L);
-    tend(kStageSort);
     if (e) return e;
     pts_set = true;
     return 0;
@@ -861,6 +893,7 @@ int Plan<T>::setpts_blocks()
 template <typename T>
 int Plan<T>::setpts(int M_, const T *x, const T *y, const T *z)
 {
+    ApiRange ar("cufinufft_setpts");
     if (M_ < 0) return kErrNdataNotValid;
     if (M_ > 0 && (!x || (dim > 1 && !y) || (dim > 2 && !z))) return 1;
     DeviceGuard dg(opts.gpu_device_id);
@@ -923,14 +956,15 @@ int Plan<T>::setpts(int M_, const T *x, const T *y, const T *z)
         cap_sub = sub_bound;
     }
     int e;
-    tbegin();
-    if (method == 1 && !opts.gpu_sort) {
-        e = run_natural_order<T>(dim, geom, M, kx, ky, kz, sb, L);
-    } else {
-        e = run_bin_sort<T>(dim, geom, nbins, M, kx, ky, kz, maxsub, opts.gpu_sort == 2,
-                            need_sub, sb, L);
+    {
+        Stage st(this, kStageSort);
+        if (method == 1 && !opts.gpu_sort) {
+            e = run_natural_order<T>(dim, geom, M, kx, ky, kz, sb, L);
+        } else {
+            e = run_bin_sort<T>(dim, geom, nbins, M, kx, ky, kz, maxsub, opts.gpu_sort == 2,
+                                need_sub, sb, L);
+        }
     }
-    tend(kStageSort);
     if (e) return e;
     pts_set = true;
     return 0;
@@ -947,6 +981,7 @@ template <typename T>
 int Plan<T>::setpts_noalloc(int M_, const T *x, const T *y, const T *z, unsigned char *scr,
                             size_t scr_bytes)
 {
+    ApiRange ar("cufinufft_setpts_noalloc");
     if (M_ < 0) return kErrNdataNotValid;
     if (M_ > 0 && (!x || (dim > 1 && !y) || (dim > 2 && !z))) return 1;
     if (method == 4) return 2;  // block copies: their count is known only after counting
@@ -983,14 +1018,15 @@ int Plan<T>::setpts_noalloc(int M_, const T *x, const T *y, const T *z, unsigned
     kz = dim > 2 ? z : nullptr;
     sub_bound = sub_need;
     int e;
-    tbegin();
-    if (method == 1 && !opts.gpu_sort) {
-        e = run_natural_order<T>(dim, geom, M, kx, ky, kz, s2, L);
-    } else {
-        e = run_bin_sort<T>(dim, geom, nbins, M, kx, ky, kz, maxsub, opts.gpu_sort == 2,
-                            need_sub, s2, L);
+    {
+        Stage st(this, kStageSort);
+        if (method == 1 && !opts.gpu_sort) {
+            e = run_natural_order<T>(dim, geom, M, kx, ky, kz, s2, L);
+        } else {
+            e = run_bin_sort<T>(dim, geom, nbins, M, kx, ky, kz, maxsub, opts.gpu_sort == 2,
+                                need_sub, s2, L);
+        }
     }
-    tend(kStageSort);
     pts_set = e == 0;
     return e;
 }
@@ -998,6 +1034,7 @@ int Plan<T>::setpts_noalloc(int M_, const T *x, const T *y, const T *z, unsigned
 template <typename T>
 int Plan<T>::execute(cpx<T> *c, cpx<T> *fk)
 {
+    ApiRange ar("cufinufft_execute");
     if (!pts_set) return 1;
     if ((M > 0 && !c) || !fk) return 1;
     DeviceGuard dg(opts.gpu_device_id);
@@ -1030,36 +1067,43 @@ int Plan<T>::execute(cpx<T> *c, cpx<T> *fk)
         cpx<T> *cb = c + (long long)b0 * M;
         cpx<T> *fb = fk + (long long)b0 * ntot;
         if (type == 1) {
-            tbegin();
-            e = run_zero<T>(fw, nftot * blk, L);
-            tend(kStageGrid);
+            {
+                Stage st(this, kStageGrid);
+                e = run_zero<T>(fw, nftot * blk, L);
+            }
             if (e) return e;
-            tbegin();
-            e = run_spread<T>(method, dim, sp.ns, M, sb, cb, fw, geom, kp, maxsub,
-                              sub_bound, blk, nftot, L);
-            tend(kStageSpreadInterp);
+            {
+                Stage st(this, kStageSpreadInterp);
+                e = run_spread<T>(method, dim, sp.ns, M, sb, cb, fw, geom, kp, maxsub,
+                                  sub_bound, blk, nftot, L);
+            }
             if (e) return e;
-            tbegin();
-            e = fft(fw, blk);
-            tend(kStageFft);
+            {
+                Stage st(this, kStageFft);
+                e = fft(fw, blk);
+            }
             if (e) return e;
-            tbegin();
-            e = run_deconvolve<T>(dim, ms, nf, fw, fb, kers, blk, L);
-            tend(kStageDeconv);
+            {
+                Stage st(this, kStageDeconv);
+                e = run_deconvolve<T>(dim, ms, nf, fw, fb, kers, blk, L);
+            }
             if (e) return e;
         } else {
-            tbegin();
-            e = run_amplify<T>(dim, ms, nf, fw, fb, kers, blk, L);
-            tend(kStageDeconv);
+            {
+                Stage st(this, kStageDeconv);
+                e = run_amplify<T>(dim, ms, nf, fw, fb, kers, blk, L);
+            }
             if (e) return e;
-            tbegin();
-            e = fft(fw, blk);
-            tend(kStageFft);
+            {
+                Stage st(this, kStageFft);
+                e = fft(fw, blk);
+            }
             if (e) return e;
-            tbegin();
-            e = run_interp<T>(method, dim, sp.ns, M, sb, cb, fw, geom, kp, maxsub,
-                              sub_bound, blk, nftot, L);
-            tend(kStageSpreadInterp);
+            {
+                Stage st(this, kStageSpreadInterp);
+                e = run_interp<T>(method, dim, sp.ns, M, sb, cb, fw, geom, kp, maxsub,
+                                  sub_bound, blk, nftot, L);
+            }
             if (e) return e;
         }
     }
@@ -1075,6 +1119,7 @@ int Plan<T>::execute(cpx<T> *c, cpx<T> *fk)
 template <typename T>
 int Plan<T>::execute_grad(cpx<T> *c, cpx<T> *dc, cpx<T> *fk)
 {
+    ApiRange ar("cufinufft_execute_grad");
     if (type != 2 || !pts_set || opts.gpu_spreadinterponly) return 1;
     if ((M > 0 && (!c || !dc)) || !fk) return 1;
     DeviceGuard dg(opts.gpu_device_id);
@@ -1087,19 +1132,22 @@ int Plan<T>::execute_grad(cpx<T> *c, cpx<T> *dc, cpx<T> *fk)
         cpx<T> *out = comp == 0 ? c : dc + (long long)(comp - 1) * plane;
         for (int b0 = 0; b0 < ntransf; b0 += maxbatch) {
             const int blk = std::min(maxbatch, ntransf - b0);
-            tbegin();
-            e = run_amplify<T>(dim, ms, nf, fw, fk + (long long)b0 * ntot, kers, blk, L, comp - 1,
-                               iflag);
-            tend(kStageDeconv);
+            {
+                Stage st(this, kStageDeconv);
+                e = run_amplify<T>(dim, ms, nf, fw, fk + (long long)b0 * ntot, kers, blk, L, comp - 1,
+                                   iflag);
+            }
             if (e) return e;
-            tbegin();
-            e = fft(fw, blk);
-            tend(kStageFft);
+            {
+                Stage st(this, kStageFft);
+                e = fft(fw, blk);
+            }
             if (e) return e;
-            tbegin();
-            e = run_interp<T>(method, dim, sp.ns, M, sb, out + (long long)b0 * M, fw, geom, kp,
-                              maxsub, sub_bound, blk, nftot, L);
-            tend(kStageSpreadInterp);
+            {
+                Stage st(this, kStageSpreadInterp);
+                e = run_interp<T>(method, dim, sp.ns, M, sb, out + (long long)b0 * M, fw, geom, kp,
+                                  maxsub, sub_bound, blk, nftot, L);
+            }
             if (e) return e;
         }
     }
@@ -1310,6 +1358,7 @@ struct Group {
 
     int execute(int use_graph)
     {
+        ApiRange ar("cufinufft_group_execute");
         DeviceGuard dg(dev);
         if (hipEventRecord(ev_in, user) != hipSuccess ||
             hipStreamWaitEvent(stream, ev_in, 0) != hipSuccess)
@@ -1362,6 +1411,7 @@ template <typename P, typename T>
 int makeplan_impl(int type, int dim, int *nmodes, int iflag, int ntransf, T tol, int maxbatch,
                   P **out, cufinufft_opts *opts, int flags = 0)
 {
+    cfa::ApiRange ar("cufinufft_makeplan");
     if (!out) return 1;
     *out = nullptr;
     P *p = new (std::nothrow) P();

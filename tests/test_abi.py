@@ -132,3 +132,17 @@ def test_python_class_option_errors():
     assert "Invalid option 'foo'" in err.value.args[0]
     with pytest.raises(TypeError):
         cufinufft(1, (8, 8), dtype=np.int32)
+
+
+def test_roctx_ranges_linked(L):
+    """makeplan / setpts / execute open roctx ranges (the reference's NVTX
+    ranges, include/profile.h:14-15): the library imports the push / pop
+    entry points from the roctx library it links."""
+    import subprocess
+    path = L.lib._name
+    dyn = subprocess.run(["nm", "-D", "--undefined-only", path], capture_output=True,
+                         text=True, check=True).stdout
+    assert "roctxRangePushA" in dyn and "roctxRangePop" in dyn
+    needed = subprocess.run(["readelf", "-d", path], capture_output=True, text=True,
+                            check=True).stdout
+    assert "librocprofiler-sdk-roctx" in needed
