@@ -95,6 +95,7 @@ _make_planf_ext.restype = c_int
 
 ACCURACY_FIRST = 1
 TILE_SLOTS = 2
+FULL_FFT = 4
 
 _set_pts = lib.cufinufft_setpts
 _set_pts.argtypes = [c_int, c_void_p, c_void_p, c_void_p, c_int, c_double_p,

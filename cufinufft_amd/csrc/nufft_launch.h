@@ -204,6 +204,173 @@ int run_amplify(int dim, const int *ms, const int *nf, cpx<T> *fw, const cpx<T> 
 template <typename T>
 int run_zero(cpx<T> *p, long long n, const Launch &L);
 
+// Pruned 3D FFT transposes (fft_prune.hip).  Gather: out[t][c'][p][r] =
+// in[t][p][r][src(c')] over the Nc mode positions c' of an axis of C cells
+// (c' < kneg: the kneg negative frequencies at the top of the axis), r the
+// output's unit-stride axis, c the input's.  Scatter: the inverse, writing
+// every c < C (zeros between the modes).  hr / hp: the mode-index offsets
+// (ms/2) of r and p where the Fourier-series tables are applied.
+struct PruneGather {
+    long long in_t = 0, ip = 0, ir = 0;    // input strides (c unit)
+    long long out_t = 0, oc = 0, op = 0;   // output strides (r unit)
+    int R = 0, P = 0, Nc = 0, C = 0, kneg = 0, hr = 0, hp = 0;
+};
+struct PruneScatter {
+    long long in_t = 0, ic = 0, ip = 0;    // input strides (r unit)
+    long long out_t = 0, op = 0, orr = 0;  // output strides (c unit)
+    int R = 0, P = 0, Nc = 0, C = 0, kneg = 0, hr = 0, hp = 0;
+};
+// Cell of mode index q (frequency q - kneg) on an axis of C cells, and the
+// mode index of cell c (-1 where the cell holds no mode); Nc modes, the kneg
+// negative frequencies at the top of the axis.
+__host__ __device__ inline int prune_cell(int q, int kneg, int C)
+{
+    return q < kneg ? C - kneg + q : q - kneg;
+}
+__host__ __device__ inline int prune_mode(int c, int Nc, int kneg, int C)
+{
+    return c < Nc - kneg ? c + kneg : (c >= C - kneg ? c - C + kneg : -1);
+}
+
+// The transposes of the pruned 3D FFT for modes ms and fine grid nf (x
+// fastest everywhere).  Buffers: fw [z][y][x] (nf0 nf1 nf2 per transform),
+// B1 [x'][z][y] (ms0 nf2 nf1), B2 [x'][y'][z] (ms0 ms1 nf2, held in fw),
+// fk [kz][ky][kx] (ms0 ms1 ms2); x', y', kx, ky, kz are mode indices.
+// Type 1: fw -> B1 (x pass done), B1 -> B2 (y pass done), B2 -> fk (z pass
+// done; with the Fourier-series tables: the deconvolve).
+inline PruneGather prune_fw_to_b1(const int *ms, const int *nf)
+{
+    PruneGather d;
+    d.in_t = (long long)nf[0] * nf[1] * nf[2];
+    d.ip = (long long)nf[1] * nf[0];
+    d.ir = nf[0];
+    d.out_t = (long long)ms[0] * nf[2] * nf[1];
+    d.oc = (long long)nf[2] * nf[1];
+    d.op = nf[1];
+    d.R = nf[1];
+    d.P = nf[2];
+    d.Nc = ms[0];
+    d.C = nf[0];
+    d.kneg = ms[0] / 2;
+    return d;
+}
+inline PruneGather prune_b1_to_b2(const int *ms, const int *nf)
+{
+    PruneGather d;
+    d.in_t = (long long)ms[0] * nf[2] * nf[1];
+    d.ip = (long long)nf[2] * nf[1];
+    d.ir = nf[1];
+    d.out_t = (long long)ms[0] * ms[1] * nf[2];
+    d.oc = nf[2];
+    d.op = (long long)ms[1] * nf[2];
+    d.R = nf[2];
+    d.P = ms[0];
+    d.Nc = ms[1];
+    d.C = nf[1];
+    d.kneg = ms[1] / 2;
+    return d;
+}
+inline PruneGather prune_b2_to_fk(const int *ms, const int *nf)
+{
+    PruneGather d;
+    d.in_t = (long long)ms[0] * ms[1] * nf[2];
+    d.ip = nf[2];
+    d.ir = (long long)ms[1] * nf[2];
+    d.out_t = (long long)ms[0] * ms[1] * ms[2];
+    d.oc = (long long)ms[1] * ms[0];
+    d.op = ms[0];
+    d.R = ms[0];
+    d.P = ms[1];
+    d.Nc = ms[2];
+    d.C = nf[2];
+    d.kneg = ms[2] / 2;
+    d.hr = ms[0] / 2;
+    d.hp = ms[1] / 2;
+    return d;
+}
+// Type 2: fk -> B2 (with the tables: the amplify; z then transformed),
+// B2 -> B1 (y then transformed), B1 -> fw (x then transformed).
+inline PruneScatter prune_fk_to_b2(const int *ms, const int *nf)
+{
+    PruneScatter d;
+    d.in_t = (long long)ms[0] * ms[1] * ms[2];
+    d.ic = (long long)ms[1] * ms[0];
+    d.ip = ms[0];
+    d.out_t = (long long)ms[0] * ms[1] * nf[2];
+    d.op = nf[2];
+    d.orr = (long long)ms[1] * nf[2];
+    d.R = ms[0];
+    d.P = ms[1];
+    d.Nc = ms[2];
+    d.C = nf[2];
+    d.kneg = ms[2] / 2;
+    d.hr = ms[0] / 2;
+    d.hp = ms[1] / 2;
+    return d;
+}
+inline PruneScatter prune_b2_to_b1(const int *ms, const int *nf)
+{
+    PruneScatter d;
+    d.in_t = (long long)ms[0] * ms[1] * nf[2];
+    d.ic = nf[2];
+    d.ip = (long long)ms[1] * nf[2];
+    d.out_t = (long long)ms[0] * nf[2] * nf[1];
+    d.op = (long long)nf[2] * nf[1];
+    d.orr = nf[1];
+    d.R = nf[2];
+    d.P = ms[0];
+    d.Nc = ms[1];
+    d.C = nf[1];
+    d.kneg = ms[1] / 2;
+    return d;
+}
+inline PruneScatter prune_b1_to_fw(const int *ms, const int *nf)
+{
+    PruneScatter d;
+    d.in_t = (long long)ms[0] * nf[2] * nf[1];
+    d.ic = (long long)nf[2] * nf[1];
+    d.ip = nf[1];
+    d.out_t = (long long)nf[0] * nf[1] * nf[2];
+    d.op = (long long)nf[1] * nf[0];
+    d.orr = nf[0];
+    d.R = nf[1];
+    d.P = nf[2];
+    d.Nc = ms[0];
+    d.C = nf[0];
+    d.kneg = ms[0] / 2;
+    return d;
+}
+// Element offsets the transposes read and write: gather (t, p, r, mode c'),
+// scatter (t, p, r, mode c') read / (t, p, r, cell c) write.
+__host__ __device__ inline long long prune_gather_src(const PruneGather &d, long long t, int p,
+                                                      int r, int q)
+{
+    return t * d.in_t + p * d.ip + r * d.ir + prune_cell(q, d.kneg, d.C);
+}
+__host__ __device__ inline long long prune_gather_dst(const PruneGather &d, long long t, int p,
+                                                      int r, int q)
+{
+    return t * d.out_t + q * d.oc + p * d.op + r;
+}
+__host__ __device__ inline long long prune_scatter_src(const PruneScatter &d, long long t, int p,
+                                                       int r, int q)
+{
+    return t * d.in_t + q * d.ic + p * d.ip + r;
+}
+__host__ __device__ inline long long prune_scatter_dst(const PruneScatter &d, long long t, int p,
+                                                       int r, int c)
+{
+    return t * d.out_t + p * d.op + r * d.orr + c;
+}
+
+template <typename T>
+int run_prune_gather(const cpx<T> *in, cpx<T> *out, const PruneGather &d, int ntr,
+                     const T *kr, const T *kp, const T *kc, const Launch &L);
+template <typename T>
+int run_prune_scatter(const cpx<T> *in, cpx<T> *out, const PruneScatter &d, int ntr,
+                      const T *kr, const T *kp, const T *kc, int wsel, int wsign,
+                      const Launch &L);
+
 // Plan groups: one persistent spread (type 1) or interp (type 2) launch over
 // the subproblems of every member.  items is a device array of n
 // GroupItem<T> (filled on the host with group_item_fill and copied), off a

@@ -438,6 +438,15 @@ struct Plan {
     hipfftHandle fft_full = 0, fft_rem = 0;
     int fft_rem_batch = 0;
     bool fft_full_ok = false, fft_rem_ok = false;
+    // Pruned 3D FFT (fft_prune.hip): one contiguous batched 1D hipFFT per
+    // axis, over the lines that hold modes, with the transposes between them
+    // going through fwh ([batch][ms0][nf2][nf1]); handles per batch size.
+    bool pruned = false;
+    cpx<T> *fwh = nullptr;
+    struct Fft1d {
+        hipfftHandle h[3] = {0, 0, 0};
+        int batch = -1;
+    } pf_full, pf_rem;
     Launch L;
 
     // Stage timer: HIP events recorded on the plan stream around every stage
@@ -508,7 +517,10 @@ struct Plan {
         if (fft_full_ok) hipfftDestroy(fft_full);
         if (fft_rem_ok) hipfftDestroy(fft_rem);
         fft_full_ok = fft_rem_ok = false;
+        drop_fft1d(pf_full);
+        drop_fft1d(pf_rem);
         dfree(fw);
+        dfree(fwh);
         for (auto &k : fwker) dfree(k);
         dfree(hcoef);
         dfree(sb.binsize);
@@ -569,6 +581,111 @@ struct Plan {
         }
         if (hipfftSetStream(h, L.stream) != HIPFFT_SUCCESS) return kErrFft;
         return FftType<T>::exec(h, data, iflag) == HIPFFT_SUCCESS ? 0 : kErrFft;
+    }
+
+    static void drop_fft1d(Fft1d &f)
+    {
+        for (auto &h : f.h) {
+            if (h) hipfftDestroy(h);
+            h = 0;
+        }
+        f.batch = -1;
+    }
+    // The three 1D plans of the pruned FFT for `batch` transforms: x over
+    // every grid line, y over the ms0 mode columns, z over the ms0 x ms1
+    // mode columns (each line contiguous, lines nf apart).
+    int make_fft1d(Fft1d &f, int batch)
+    {
+        drop_fft1d(f);
+        const long long lines[3] = {(long long)nf[1] * nf[2], (long long)ms[0] * nf[2],
+                                    (long long)ms[0] * ms[1]};
+        for (int a = 0; a < 3; ++a) {
+            int n = nf[a];
+            if (hipfftCreate(&f.h[a]) != HIPFFT_SUCCESS) {
+                f.h[a] = 0;
+                drop_fft1d(f);
+                return kErrFft;
+            }
+            if (hipfftPlanMany(&f.h[a], 1, &n, &n, 1, n, &n, 1, n, FftType<T>::type,
+                               (int)(lines[a] * batch)) != HIPFFT_SUCCESS) {
+                drop_fft1d(f);
+                return kErrFft;
+            }
+        }
+        f.batch = batch;
+        return 0;
+    }
+    int fft1d(Fft1d *&f, int batch)
+    {
+        if (batch == pf_full.batch) {
+            f = &pf_full;
+            return 0;
+        }
+        if (batch != pf_rem.batch) {
+            const int e = make_fft1d(pf_rem, batch);
+            if (e) return e;
+        }
+        f = &pf_rem;
+        return 0;
+    }
+    int exec1d(hipfftHandle h, cpx<T> *data)
+    {
+        if (hipfftSetStream(h, L.stream) != HIPFFT_SUCCESS) return kErrFft;
+        return FftType<T>::exec(h, data, iflag) == HIPFFT_SUCCESS ? 0 : kErrFft;
+    }
+
+    // Type 1 after the spread: the FFT of the fine grid in fw.  Pruned, it
+    // leaves the ms0 x ms1 x nf2 block [x'][y'][z] (mode columns, z
+    // transformed) at the start of each transform's fw.
+    int fft_type1(int blk)
+    {
+        if (!pruned) return fft(fw, blk);
+        Fft1d *f = nullptr;
+        int e = fft1d(f, blk);
+        if (e) return e;
+        if ((e = exec1d(f->h[0], fw))) return e;
+        const auto g1 = prune_fw_to_b1(ms, nf);
+        if ((e = run_prune_gather<T>(fw, fwh, g1, blk, nullptr, nullptr, nullptr, L))) return e;
+        if ((e = exec1d(f->h[1], fwh))) return e;
+        const auto g2 = prune_b1_to_b2(ms, nf);
+        if ((e = run_prune_gather<T>(fwh, fw, g2, blk, nullptr, nullptr, nullptr, L))) return e;
+        return exec1d(f->h[2], fw);
+    }
+    // Type 1, last step: fk = the modes of fw divided by the kernel's
+    // Fourier series (the pruned form reads the block fft_type1 leaves).
+    int deconv_type1(cpx<T> *fk, int blk)
+    {
+        const T *kers[3] = {fwker[0], fwker[1], fwker[2]};
+        if (!pruned) return run_deconvolve<T>(dim, ms, nf, fw, fk, kers, blk, L);
+        const auto d = prune_b2_to_fk(ms, nf);
+        return run_prune_gather<T>(fw, fk, d, blk, kers[0], kers[1], kers[2], L);
+    }
+    // Type 2, first step: the modes fk divided by the kernel's Fourier
+    // series (times i * wsign * k_wax for wax >= 0), zero-padded into fw
+    // (pruned: into the ms0 x ms1 x nf2 block [x'][y'][z]).
+    int amplify_type2(const cpx<T> *fk, int blk, int wax, int wsign)
+    {
+        const T *kers[3] = {fwker[0], fwker[1], fwker[2]};
+        if (!pruned) return run_amplify<T>(dim, ms, nf, fw, fk, kers, blk, L, wax, wsign);
+        const auto d = prune_fk_to_b2(ms, nf);
+        return run_prune_scatter<T>(fk, fw, d, blk, kers[0], kers[1], kers[2], wax, wsign, L);
+    }
+    // Type 2: the FFT that turns what amplify_type2 wrote into the fine grid.
+    int fft_type2(int blk)
+    {
+        if (!pruned) return fft(fw, blk);
+        Fft1d *f = nullptr;
+        int e = fft1d(f, blk);
+        if (e) return e;
+        if ((e = exec1d(f->h[2], fw))) return e;
+        const auto s1 = prune_b2_to_b1(ms, nf);
+        if ((e = run_prune_scatter<T>(fw, fwh, s1, blk, nullptr, nullptr, nullptr, -1, 1, L)))
+            return e;
+        if ((e = exec1d(f->h[1], fwh))) return e;
+        const auto s2 = prune_b1_to_fw(ms, nf);
+        if ((e = run_prune_scatter<T>(fwh, fw, s2, blk, nullptr, nullptr, nullptr, -1, 1, L)))
+            return e;
+        return exec1d(f->h[0], fw);
     }
 
     int makeplan(int type_, int dim_, const int *nmodes, int iflag_, int ntransf_, T tol,
@@ -645,7 +762,8 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
 {
     if (type_ != 1 && type_ != 2) return 1;
     if (dim_ < 1 || dim_ > 3) return 1;
-    if (flags & ~(CUFINUFFT_ACCURACY_FIRST | CUFINUFFT_TILE_SLOTS)) return 1;
+    if (flags & ~(CUFINUFFT_ACCURACY_FIRST | CUFINUFFT_TILE_SLOTS | CUFINUFFT_FULL_FFT))
+        return 1;
     // Accuracy-first: the reference's width rule ns = ceil(-log10(tol/10))
     // leaves the transform's relative l2 error at 0.6-1.1, 1-2 and 1.5-3.2
     // times tol in 1, 2 and 3 dimensions (the same algorithm in FP64 against
@@ -784,9 +902,22 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
                 return kErrHip;
         }
         if (dmalloc(&fw, (size_t)(nftot * maxbatch))) return kErrHip;
-        int e = make_fft(&fft_full, maxbatch);
-        if (e) return e;
-        fft_full_ok = true;
+        // 3D grids take the pruned FFT unless the caller asks for the full
+        // one; its launch grids and hipFFT batch counts bound the sizes.
+        pruned = dim == 3 && !(flags & CUFINUFFT_FULL_FFT) && maxbatch <= 65535 &&
+                 nf[1] <= 65535 && nf[2] <= 65535 &&
+                 (long long)nf[1] * nf[2] * maxbatch <= 2147483647LL &&
+                 (long long)ms[0] * nf[2] * nf[1] * maxbatch <= 2147483647LL;
+        if (pruned) {
+            if (dmalloc(&fwh, (size_t)((long long)ms[0] * nf[2] * nf[1] * maxbatch)))
+                return kErrHip;
+            int e = make_fft1d(pf_full, maxbatch);
+            if (e) return e;
+        } else {
+            int e = make_fft(&fft_full, maxbatch);
+            if (e) return e;
+            fft_full_ok = true;
+        }
     }
     if (opts.gpu_kerevalmeth == 1) {
         hdeg = horner_fit(sp.ns, (double)sp.beta, (double)sp.c, (double)tol, hcoef_host, &herr);
@@ -1040,7 +1171,6 @@ int Plan<T>::execute(cpx<T> *c, cpx<T> *fk)
     DeviceGuard dg(opts.gpu_device_id);
     const KerParams<T> kp = ker_params();
     const int maxsub = opts.gpu_maxsubprobsize;
-    const T *kers[3] = {fwker[0], fwker[1], fwker[2]};
     int e = 0;
     if (opts.gpu_spreadinterponly) {
         // The fine grid is the caller's fk (nf = nmodes): spread into it or
@@ -1080,23 +1210,23 @@ int Plan<T>::execute(cpx<T> *c, cpx<T> *fk)
             if (e) return e;
             {
                 Stage st(this, kStageFft);
-                e = fft(fw, blk);
+                e = fft_type1(blk);
             }
             if (e) return e;
             {
                 Stage st(this, kStageDeconv);
-                e = run_deconvolve<T>(dim, ms, nf, fw, fb, kers, blk, L);
+                e = deconv_type1(fb, blk);
             }
             if (e) return e;
         } else {
             {
                 Stage st(this, kStageDeconv);
-                e = run_amplify<T>(dim, ms, nf, fw, fb, kers, blk, L);
+                e = amplify_type2(fb, blk, -1, 1);
             }
             if (e) return e;
             {
                 Stage st(this, kStageFft);
-                e = fft(fw, blk);
+                e = fft_type2(blk);
             }
             if (e) return e;
             {
@@ -1125,7 +1255,6 @@ int Plan<T>::execute_grad(cpx<T> *c, cpx<T> *dc, cpx<T> *fk)
     DeviceGuard dg(opts.gpu_device_id);
     const KerParams<T> kp = ker_params();
     const int maxsub = opts.gpu_maxsubprobsize;
-    const T *kers[3] = {fwker[0], fwker[1], fwker[2]};
     const long long plane = (long long)M * ntransf;
     int e = 0;
     for (int comp = 0; comp <= dim; ++comp) {  // 0: values, 1 + d: d/dx_d
@@ -1134,13 +1263,12 @@ int Plan<T>::execute_grad(cpx<T> *c, cpx<T> *dc, cpx<T> *fk)
             const int blk = std::min(maxbatch, ntransf - b0);
             {
                 Stage st(this, kStageDeconv);
-                e = run_amplify<T>(dim, ms, nf, fw, fk + (long long)b0 * ntot, kers, blk, L, comp - 1,
-                                   iflag);
+                e = amplify_type2(fk + (long long)b0 * ntot, blk, comp - 1, iflag);
             }
             if (e) return e;
             {
                 Stage st(this, kStageFft);
-                e = fft(fw, blk);
+                e = fft_type2(blk);
             }
             if (e) return e;
             {
@@ -1336,18 +1464,17 @@ struct Group {
             if (!e)
                 e = branches([&](int i, const Launch &Lb) {
                     Plan<T> &p = *plans[i];
-                    const T *kers[3] = {p.fwker[0], p.fwker[1], p.fwker[2]};
-                    int err = p.fft(p.fw, p.ntransf);
-                    if (!err)
-                        err = run_deconvolve<T>(dim, p.ms, p.nf, p.fw, fk[i], kers, p.ntransf, Lb);
+                    (void)Lb;  // the member's own launches follow its branch stream
+                    int err = p.fft_type1(p.ntransf);
+                    if (!err) err = p.deconv_type1(fk[i], p.ntransf);
                     return err;
                 });
         } else {
             e = branches([&](int i, const Launch &Lb) {
                 Plan<T> &p = *plans[i];
-                const T *kers[3] = {p.fwker[0], p.fwker[1], p.fwker[2]};
-                int err = run_amplify<T>(dim, p.ms, p.nf, p.fw, fk[i], kers, p.ntransf, Lb);
-                if (!err) err = p.fft(p.fw, p.ntransf);
+                (void)Lb;  // the member's own launches follow its branch stream
+                int err = p.amplify_type2(fk[i], p.ntransf, -1, 1);
+                if (!err) err = p.fft_type2(p.ntransf);
                 return err;
             });
             if (!e) e = spread_or_interp();

@@ -105,16 +105,16 @@ def test_makeplan_error_codes(L, kw, code):
 
 
 def test_makeplan_ext_flags(L):
-    """Planning flags: 1 accuracy-first, 2 tile slots; any other bit is
-    rejected with 1 before the device is touched."""
+    """Planning flags: 1 accuracy-first, 2 tile slots, 4 full FFT; any other
+    bit is rejected with 1 before the device is touched."""
     o = L.NufftOpts()
     L._default_opts(1, 2, o)
     plan = ctypes.c_void_p()
     modes = (ctypes.c_int * 3)(16, 16, 1)
-    assert L._make_planf_ext(1, 2, modes, 1, 1, 1e-6, 1, ctypes.byref(plan), o, 4) == 1
-    assert L._make_planf_ext(1, 2, modes, 1, 1, 1e-6, 1, ctypes.byref(plan), o, 8 | 1) == 1
+    assert L._make_planf_ext(1, 2, modes, 1, 1, 1e-6, 1, ctypes.byref(plan), o, 8) == 1
+    assert L._make_planf_ext(1, 2, modes, 1, 1, 1e-6, 1, ctypes.byref(plan), o, 16 | 1) == 1
     assert not plan.value
-    assert L.TILE_SLOTS == 2 and L.ACCURACY_FIRST == 1
+    assert L.TILE_SLOTS == 2 and L.ACCURACY_FIRST == 1 and L.FULL_FFT == 4
 
 
 def test_null_plan_calls(L):
