@@ -37,45 +37,61 @@ template <typename T> constexpr int prune_w() { return sizeof(T) == 4 ? 64 : 32;
 // src(c') = C - kneg + c' for c' < kneg (negative frequencies, at the top of
 // the axis) and c' - kneg otherwise; s = 1 / (kr[|r-hr|] kp[|p-hp|] kc[|c'-kneg|])
 // when the tables are given (the type-1 deconvolve, as k_deconvolve), else 1.
-template <typename T, int W>
+template <typename T, int W, bool SCALE>
 __global__ void __launch_bounds__(256)
 k_prune_gather(const cpx<T> *__restrict__ in, cpx<T> *__restrict__ out, PruneGather d,
                const T *__restrict__ kr, const T *__restrict__ kp, const T *__restrict__ kc)
 {
     __shared__ cpx<T> tile[W][W + 1];
-    constexpr int ROWS = 256 / W;
+    constexpr int ROWS = 256 / W, PER = W / ROWS;
     const int nrt = (d.R + W - 1) / W;
     const int r0 = (int)(blockIdx.x % nrt) * W, c0 = (int)(blockIdx.x / nrt) * W;
     const int p = blockIdx.y;
     const long long t = blockIdx.z;
     const int tx = threadIdx.x % W, ty = threadIdx.x / W;
+    // all PER loads of a thread in flight together, then into the tile
     {
         const int cq = c0 + tx;
-        if (cq < d.Nc) {
-            for (int i = ty; i < W; i += ROWS) {
-                const int r = r0 + i;
-                if (r < d.R) tile[tx][i] = in[prune_gather_src(d, t, p, r, cq)];
-            }
+        cpx<T> v[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int r = r0 + ty + k * ROWS;
+            v[k].x = 0;
+            v[k].y = 0;
+            if (cq < d.Nc && r < d.R) v[k] = in[prune_gather_src(d, t, p, r, cq)];
         }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) tile[tx][ty + k * ROWS] = v[k];
     }
     __syncthreads();
     const int r = r0 + tx;
     if (r >= d.R) return;
     T sr = 1, sp = 1;
-    if (kr) {
+    if (SCALE) {
         sr = kr[abs(r - d.hr)];
         sp = kp[abs(p - d.hp)];
     }
-    for (int j = ty; j < W; j += ROWS) {
-        const int c = c0 + j;
-        if (c >= d.Nc) break;
-        cpx<T> v = tile[j][tx];
-        if (kr) {
-            const T kv = sr * (sp * kc[abs(c - d.kneg)]);
-            v.x = v.x / kv;
-            v.y = v.y / kv;
+    cpx<T> v[PER];
+    T kv[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int c = c0 + ty + k * ROWS;
+        v[k] = tile[ty + k * ROWS][tx];
+        kv[k] = 1;
+        // the mode index is clamped so every table read is in bounds
+        if (SCALE) kv[k] = sr * (sp * kc[abs(min(c, d.Nc - 1) - d.kneg)]);
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int c = c0 + ty + k * ROWS;
+        if (c < d.Nc) {
+            cpx<T> o = v[k];
+            if (SCALE) {
+                o.x = o.x / kv[k];
+                o.y = o.y / kv[k];
+            }
+            out[prune_gather_dst(d, t, p, r, c)] = o;
         }
-        out[prune_gather_dst(d, t, p, r, c)] = v;
     }
 }
 
@@ -84,14 +100,14 @@ k_prune_gather(const cpx<T> *__restrict__ in, cpx<T> *__restrict__ out, PruneGat
 // 1 / the kernel's Fourier series product (the type-2 amplify, as k_amplify)
 // and, for wsel 0 / 1 / 2, w = i * wsign * the frequency along r / p / c (the
 // derivative modes of execute_grad); else s = w = 1.
-template <typename T, int W>
+template <typename T, int W, bool SCALE>
 __global__ void __launch_bounds__(256)
 k_prune_scatter(const cpx<T> *__restrict__ in, cpx<T> *__restrict__ out, PruneScatter d,
                 const T *__restrict__ kr, const T *__restrict__ kp, const T *__restrict__ kc,
                 int wsel, int wsign)
 {
     __shared__ cpx<T> tile[W][W + 1];
-    constexpr int ROWS = 256 / W;
+    constexpr int ROWS = 256 / W, PER = W / ROWS;
     const int nrt = (d.R + W - 1) / W;
     const int r0 = (int)(blockIdx.x % nrt) * W, c0 = (int)(blockIdx.x / nrt) * W;
     const int p = blockIdx.y;
@@ -100,41 +116,53 @@ k_prune_scatter(const cpx<T> *__restrict__ in, cpx<T> *__restrict__ out, PruneSc
     {
         const int r = r0 + tx;
         T sr = 1, sp = 1;
-        if (kr && r < d.R) {
+        if (SCALE && r < d.R) {
             sr = kr[abs(r - d.hr)];
             sp = kp[abs(p - d.hp)];
         }
-        for (int j = ty; j < W; j += ROWS) {
-            const int c = c0 + j;
-            cpx<T> v;
-            v.x = 0;
-            v.y = 0;
-            const int cq = c < d.C ? prune_mode(c, d.Nc, d.kneg, d.C) : -1;
-            if (cq >= 0 && r < d.R) {
-                v = in[prune_scatter_src(d, t, p, r, cq)];
-                if (kr) {
-                    const T kv = sr * (sp * kc[abs(cq - d.kneg)]);
-                    v.x = v.x / kv;
-                    v.y = v.y / kv;
-                    if (wsel >= 0) {
-                        const int f = wsel == 0 ? r - d.hr : (wsel == 1 ? p - d.hp : cq - d.kneg);
-                        const T w = (T)(wsign * f);
-                        const T re = -v.y * w;
-                        v.y = v.x * w;
-                        v.x = re;
-                    }
+        cpx<T> v[PER];
+        int q[PER];
+        T kv[PER];
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int c = c0 + ty + k * ROWS;
+            q[k] = c < d.C && r < d.R ? prune_mode(c, d.Nc, d.kneg, d.C) : -1;
+            v[k].x = 0;
+            v[k].y = 0;
+            kv[k] = 1;
+            if (q[k] >= 0) {
+                v[k] = in[prune_scatter_src(d, t, p, r, q[k])];
+                if (SCALE) kv[k] = sr * (sp * kc[abs(q[k] - d.kneg)]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            cpx<T> o = v[k];
+            if (SCALE && q[k] >= 0) {
+                o.x = o.x / kv[k];
+                o.y = o.y / kv[k];
+                if (wsel >= 0) {
+                    const int f =
+                        wsel == 0 ? r - d.hr : (wsel == 1 ? p - d.hp : q[k] - d.kneg);
+                    const T w = (T)(wsign * f);
+                    const T re = -o.y * w;
+                    o.y = o.x * w;
+                    o.x = re;
                 }
             }
-            tile[j][tx] = v;
+            tile[ty + k * ROWS][tx] = o;
         }
     }
     __syncthreads();
     const int c = c0 + tx;
     if (c >= d.C) return;
-    for (int i = ty; i < W; i += ROWS) {
-        const int r = r0 + i;
-        if (r >= d.R) break;
-        out[prune_scatter_dst(d, t, p, r, c)] = tile[tx][i];
+    cpx<T> v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) v[k] = tile[tx][ty + k * ROWS];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int r = r0 + ty + k * ROWS;
+        if (r < d.R) out[prune_scatter_dst(d, t, p, r, c)] = v[k];
     }
 }
 
@@ -147,7 +175,10 @@ int run_prune_gather(const cpx<T> *in, cpx<T> *out, const PruneGather &d, int nt
     if (d.P > 65535 || ntr > 65535) return kErrHip;
     const long long nt = (long long)((d.R + W - 1) / W) * ((d.Nc + W - 1) / W);
     dim3 grid((unsigned)nt, (unsigned)d.P, (unsigned)ntr);
-    k_prune_gather<T, W><<<grid, 256, 0, L.stream>>>(in, out, d, kr, kp, kc);
+    if (kr && kp && kc)
+        k_prune_gather<T, W, true><<<grid, 256, 0, L.stream>>>(in, out, d, kr, kp, kc);
+    else
+        k_prune_gather<T, W, false><<<grid, 256, 0, L.stream>>>(in, out, d, kr, kp, kc);
     return ok_or_err();
 }
 
@@ -161,7 +192,12 @@ int run_prune_scatter(const cpx<T> *in, cpx<T> *out, const PruneScatter &d, int 
     if (d.P > 65535 || ntr > 65535) return kErrHip;
     const long long nt = (long long)((d.R + W - 1) / W) * ((d.C + W - 1) / W);
     dim3 grid((unsigned)nt, (unsigned)d.P, (unsigned)ntr);
-    k_prune_scatter<T, W><<<grid, 256, 0, L.stream>>>(in, out, d, kr, kp, kc, wsel, wsign);
+    if (kr && kp && kc)
+        k_prune_scatter<T, W, true><<<grid, 256, 0, L.stream>>>(in, out, d, kr, kp, kc, wsel,
+                                                                wsign);
+    else
+        k_prune_scatter<T, W, false><<<grid, 256, 0, L.stream>>>(in, out, d, kr, kp, kc, wsel,
+                                                                 wsign);
     return ok_or_err();
 }
 
