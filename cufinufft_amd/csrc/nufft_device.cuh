@@ -1069,11 +1069,13 @@ __device__ __forceinline__ void spread_sm_job(const SubprobJob &J, int M, const 
                 for (int dy = 0; dy < (DIM > 1 ? NS : 1); ++dy) {
                     const T kyz = (DIM > 1 ? k2[dy] : T(1)) * kz;
                     const T cr = sr * kyz, ci = si * kyz;
-                    const int row = s0 + (dz * tl.tw[1] + dy) * tl.tw[0];
+                    // one base per row: the taps are constant LDS offsets
+                    Acc *pr = sre + (s0 + (dz * tl.tw[1] + dy) * tl.tw[0]);
+                    Acc *pi = pr + n;
 #pragma unroll
                     for (int a = 0; a < NS; ++a) {
-                        FxAcc<Acc>::add(&sre[row + a], FxAcc<Acc>::cvt(cr * k1[a]));
-                        FxAcc<Acc>::add(&sim[row + a], FxAcc<Acc>::cvt(ci * k1[a]));
+                        FxAcc<Acc>::add(pr + a, FxAcc<Acc>::cvt(cr * k1[a]));
+                        FxAcc<Acc>::add(pi + a, FxAcc<Acc>::cvt(ci * k1[a]));
                     }
                 }
             }
