@@ -272,13 +272,42 @@ k_sort_scatter(int M, const T *__restrict__ x, const T *__restrict__ y, const T 
 
 // binsize[b] = end - start of b's run in the sorted keys (binsize zeroed
 // first; the first and last element of each run add -start and end + 1).
+// Four keys per thread (one 16-byte load, skey 16-byte aligned) and their
+// two neighbours; the division by the cells per bin only for (bin, cell)
+// keys (DIV).  A bin of -1 marks "no neighbour".
+template <bool DIV>
 __global__ void __launch_bounds__(256)
 k_run_sizes(int M, int vol, const int *__restrict__ skey, int *__restrict__ binsize)
 {
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < M; s += gridDim.x * blockDim.x) {
-        const int b = skey[s] / vol;
-        if (s == 0 || skey[s - 1] / vol != b) atomicAdd(&binsize[b], -s);
-        if (s == M - 1 || skey[s + 1] / vol != b) atomicAdd(&binsize[b], s + 1);
+    const long long n4 = ((long long)M + 3) / 4;
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n4;
+         q += (long long)gridDim.x * blockDim.x) {
+        const int s0 = (int)(q * 4);
+        int k[6];
+        if (s0 + 3 < M) {
+            const int4 v = reinterpret_cast<const int4 *>(skey)[q];
+            k[1] = v.x;
+            k[2] = v.y;
+            k[3] = v.z;
+            k[4] = v.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) k[1 + j] = s0 + j < M ? skey[s0 + j] : 0;
+        }
+        k[0] = s0 > 0 ? skey[s0 - 1] : 0;
+        k[5] = s0 + 4 < M ? skey[s0 + 4] : 0;
+        int b[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) b[j] = DIV ? k[j] / vol : k[j];
+        if (s0 == 0) b[0] = -1;
+#pragma unroll
+        for (int j = 1; j <= 4; ++j) {
+            const int s = s0 + j - 1;
+            if (s >= M) break;
+            if (s + 1 >= M) b[j + 1] = -1;
+            if (b[j - 1] != b[j]) atomicAdd(&binsize[b[j]], -s);
+            if (b[j + 1] != b[j]) atomicAdd(&binsize[b[j]], s + 1);
+        }
     }
 }
 
@@ -358,7 +387,12 @@ int run_bin_sort(int dim, const BinGeom &g, long long nbins, int M, const T *x, 
         if (e) return e;
         // the last pass wrote buffer (passes - 1) % 2
         const int *final_keys = ((sh.passes - 1) & 1) ? sb.skey_out : sb.skey;
-        k_run_sizes<<<grid_pts(M, L.ncu), 256, 0, L.stream>>>(M, vol, final_keys, sb.binsize);
+        if (reinterpret_cast<uintptr_t>(final_keys) % 16 != 0) return kErrHip;
+        const int gq = grid_pts(((long long)M + 3) / 4, L.ncu);
+        if (vol > 1)
+            k_run_sizes<true><<<gq, 256, 0, L.stream>>>(M, vol, final_keys, sb.binsize);
+        else
+            k_run_sizes<false><<<gq, 256, 0, L.stream>>>(M, vol, final_keys, sb.binsize);
     }
     const long long nparts = (nbins + kScanChunk - 1) / kScanChunk;
     int2 *part = static_cast<int2 *>(sb.scan_part);

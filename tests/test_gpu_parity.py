@@ -36,6 +36,9 @@ SORT_CASES = [
     (3, np.float32, (64, 64, 64), 200_000, "clustered", None),
     (3, np.float64, (40, 36, 50), 60_000, "uniform", (8, 8, 4)),
     (1, np.float32, (1000,), 30_000, "uniform", None),
+    # M not a multiple of 4 (the run-size pass reads keys four at a time)
+    (3, np.float32, (32, 32, 32), 4099, "clustered", (8, 8, 8)),
+    (2, np.float64, (64, 48), 7, "uniform", None),
 ]
 
 
