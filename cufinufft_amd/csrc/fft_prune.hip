@@ -36,7 +36,8 @@ template <typename T> constexpr int prune_w() { return sizeof(T) == 4 ? 64 : 32;
 // out[t][c'][p][r] = s * in[t][p][r][src(c')] for the kept c' < Nc, where
 // src(c') = C - kneg + c' for c' < kneg (negative frequencies, at the top of
 // the axis) and c' - kneg otherwise; s = 1 / (kr[|r-hr|] kp[|p-hp|] kc[|c'-kneg|])
-// when the tables are given (the type-1 deconvolve, as k_deconvolve), else 1.
+// when the tables are given (the type-1 deconvolve, as k_deconvolve; kp may
+// be null: a factor 1, the 2D case), else 1.
 template <typename T, int W, bool SCALE>
 __global__ void __launch_bounds__(256)
 k_prune_gather(const cpx<T> *__restrict__ in, cpx<T> *__restrict__ out, PruneGather d,
@@ -69,7 +70,7 @@ k_prune_gather(const cpx<T> *__restrict__ in, cpx<T> *__restrict__ out, PruneGat
     T sr = 1, sp = 1;
     if (SCALE) {
         sr = kr[abs(r - d.hr)];
-        sp = kp[abs(p - d.hp)];
+        sp = kp ? kp[abs(p - d.hp)] : T(1);
     }
     cpx<T> v[PER];
     T kv[PER];
@@ -118,7 +119,7 @@ k_prune_scatter(const cpx<T> *__restrict__ in, cpx<T> *__restrict__ out, PruneSc
         T sr = 1, sp = 1;
         if (SCALE && r < d.R) {
             sr = kr[abs(r - d.hr)];
-            sp = kp[abs(p - d.hp)];
+            sp = kp ? kp[abs(p - d.hp)] : T(1);
         }
         cpx<T> v[PER];
         int q[PER];
@@ -175,7 +176,7 @@ int run_prune_gather(const cpx<T> *in, cpx<T> *out, const PruneGather &d, int nt
     if (d.P > 65535 || ntr > 65535) return kErrHip;
     const long long nt = (long long)((d.R + W - 1) / W) * ((d.Nc + W - 1) / W);
     dim3 grid((unsigned)nt, (unsigned)d.P, (unsigned)ntr);
-    if (kr && kp && kc)
+    if (kr && kc)
         k_prune_gather<T, W, true><<<grid, 256, 0, L.stream>>>(in, out, d, kr, kp, kc);
     else
         k_prune_gather<T, W, false><<<grid, 256, 0, L.stream>>>(in, out, d, kr, kp, kc);
@@ -192,7 +193,7 @@ int run_prune_scatter(const cpx<T> *in, cpx<T> *out, const PruneScatter &d, int 
     if (d.P > 65535 || ntr > 65535) return kErrHip;
     const long long nt = (long long)((d.R + W - 1) / W) * ((d.C + W - 1) / W);
     dim3 grid((unsigned)nt, (unsigned)d.P, (unsigned)ntr);
-    if (kr && kp && kc)
+    if (kr && kc)
         k_prune_scatter<T, W, true><<<grid, 256, 0, L.stream>>>(in, out, d, kr, kp, kc, wsel,
                                                                 wsign);
     else

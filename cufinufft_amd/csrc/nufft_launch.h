@@ -340,6 +340,68 @@ inline PruneScatter prune_b1_to_fw(const int *ms, const int *nf)
     d.kneg = ms[0] / 2;
     return d;
 }
+// 2D (one plane, P = 1): fw [y][x] (nf0 nf1), B1 [x'][y] (ms0 nf1, the
+// mode columns), fk [ky][kx].  Type 1: fw -> B1 after the x pass, B1 -> fk
+// after the y pass (the deconvolve, tables kr = x and kc = y); type 2:
+// fk -> B1 (the amplify), y pass, B1 -> fw, x pass.
+inline PruneGather prune2_fw_to_b1(const int *ms, const int *nf)
+{
+    PruneGather d;
+    d.in_t = (long long)nf[0] * nf[1];
+    d.ir = nf[0];
+    d.out_t = (long long)ms[0] * nf[1];
+    d.oc = nf[1];
+    d.R = nf[1];
+    d.P = 1;
+    d.Nc = ms[0];
+    d.C = nf[0];
+    d.kneg = ms[0] / 2;
+    return d;
+}
+inline PruneGather prune2_b1_to_fk(const int *ms, const int *nf)
+{
+    PruneGather d;
+    d.in_t = (long long)ms[0] * nf[1];
+    d.ir = nf[1];
+    d.out_t = (long long)ms[0] * ms[1];
+    d.oc = ms[0];
+    d.R = ms[0];
+    d.P = 1;
+    d.Nc = ms[1];
+    d.C = nf[1];
+    d.kneg = ms[1] / 2;
+    d.hr = ms[0] / 2;
+    return d;
+}
+inline PruneScatter prune2_fk_to_b1(const int *ms, const int *nf)
+{
+    PruneScatter d;
+    d.in_t = (long long)ms[0] * ms[1];
+    d.ic = ms[0];
+    d.out_t = (long long)ms[0] * nf[1];
+    d.orr = nf[1];
+    d.R = ms[0];
+    d.P = 1;
+    d.Nc = ms[1];
+    d.C = nf[1];
+    d.kneg = ms[1] / 2;
+    d.hr = ms[0] / 2;
+    return d;
+}
+inline PruneScatter prune2_b1_to_fw(const int *ms, const int *nf)
+{
+    PruneScatter d;
+    d.in_t = (long long)ms[0] * nf[1];
+    d.ic = nf[1];
+    d.out_t = (long long)nf[0] * nf[1];
+    d.orr = nf[0];
+    d.R = nf[1];
+    d.P = 1;
+    d.Nc = ms[0];
+    d.C = nf[0];
+    d.kneg = ms[0] / 2;
+    return d;
+}
 // Element offsets the transposes read and write: gather (t, p, r, mode c'),
 // scatter (t, p, r, mode c') read / (t, p, r, cell c) write.
 __host__ __device__ inline long long prune_gather_src(const PruneGather &d, long long t, int p,

@@ -591,15 +591,16 @@ struct Plan {
         }
         f.batch = -1;
     }
-    // The three 1D plans of the pruned FFT for `batch` transforms: x over
-    // every grid line, y over the ms0 mode columns, z over the ms0 x ms1
-    // mode columns (each line contiguous, lines nf apart).
+    // The 1D plans of the pruned FFT for `batch` transforms: x over every
+    // grid line, y over the ms0 mode columns (times nf2 planes in 3D), and in
+    // 3D z over the ms0 x ms1 mode columns (each line contiguous, lines nf
+    // apart).
     int make_fft1d(Fft1d &f, int batch)
     {
         drop_fft1d(f);
         const long long lines[3] = {(long long)nf[1] * nf[2], (long long)ms[0] * nf[2],
                                     (long long)ms[0] * ms[1]};
-        for (int a = 0; a < 3; ++a) {
+        for (int a = 0; a < dim; ++a) {
             int n = nf[a];
             if (hipfftCreate(&f.h[a]) != HIPFFT_SUCCESS) {
                 f.h[a] = 0;
@@ -644,6 +645,12 @@ struct Plan {
         int e = fft1d(f, blk);
         if (e) return e;
         if ((e = exec1d(f->h[0], fw))) return e;
+        if (dim == 2) {  // leaves B1 = [x'][y] (y transformed) in fwh
+            const auto g1 = prune2_fw_to_b1(ms, nf);
+            if ((e = run_prune_gather<T>(fw, fwh, g1, blk, nullptr, nullptr, nullptr, L)))
+                return e;
+            return exec1d(f->h[1], fwh);
+        }
         const auto g1 = prune_fw_to_b1(ms, nf);
         if ((e = run_prune_gather<T>(fw, fwh, g1, blk, nullptr, nullptr, nullptr, L))) return e;
         if ((e = exec1d(f->h[1], fwh))) return e;
@@ -657,6 +664,9 @@ struct Plan {
     {
         const T *kers[3] = {fwker[0], fwker[1], fwker[2]};
         if (!pruned) return run_deconvolve<T>(dim, ms, nf, fw, fk, kers, blk, L);
+        if (dim == 2)
+            return run_prune_gather<T>(fwh, fk, prune2_b1_to_fk(ms, nf), blk, kers[0], nullptr,
+                                       kers[1], L);
         const auto d = prune_b2_to_fk(ms, nf);
         return run_prune_gather<T>(fw, fk, d, blk, kers[0], kers[1], kers[2], L);
     }
@@ -667,6 +677,9 @@ struct Plan {
     {
         const T *kers[3] = {fwker[0], fwker[1], fwker[2]};
         if (!pruned) return run_amplify<T>(dim, ms, nf, fw, fk, kers, blk, L, wax, wsign);
+        if (dim == 2)  // y is the scatter's c axis
+            return run_prune_scatter<T>(fk, fwh, prune2_fk_to_b1(ms, nf), blk, kers[0], nullptr,
+                                        kers[1], wax == 1 ? 2 : wax, wsign, L);
         const auto d = prune_fk_to_b2(ms, nf);
         return run_prune_scatter<T>(fk, fw, d, blk, kers[0], kers[1], kers[2], wax, wsign, L);
     }
@@ -677,6 +690,13 @@ struct Plan {
         Fft1d *f = nullptr;
         int e = fft1d(f, blk);
         if (e) return e;
+        if (dim == 2) {  // B1 = [x'][y] in fwh -> fw
+            if ((e = exec1d(f->h[1], fwh))) return e;
+            if ((e = run_prune_scatter<T>(fwh, fw, prune2_b1_to_fw(ms, nf), blk, nullptr, nullptr,
+                                          nullptr, -1, 1, L)))
+                return e;
+            return exec1d(f->h[0], fw);
+        }
         if ((e = exec1d(f->h[2], fw))) return e;
         const auto s1 = prune_b2_to_b1(ms, nf);
         if ((e = run_prune_scatter<T>(fw, fwh, s1, blk, nullptr, nullptr, nullptr, -1, 1, L)))
@@ -904,7 +924,7 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
         if (dmalloc(&fw, (size_t)(nftot * maxbatch))) return kErrHip;
         // 3D grids take the pruned FFT unless the caller asks for the full
         // one; its launch grids and hipFFT batch counts bound the sizes.
-        pruned = dim == 3 && !(flags & CUFINUFFT_FULL_FFT) && maxbatch <= 65535 &&
+        pruned = dim >= 2 && !(flags & CUFINUFFT_FULL_FFT) && maxbatch <= 65535 &&
                  nf[1] <= 65535 && nf[2] <= 65535 &&
                  (long long)nf[1] * nf[2] * maxbatch <= 2147483647LL &&
                  (long long)ms[0] * nf[2] * nf[1] * maxbatch <= 2147483647LL;

@@ -89,14 +89,14 @@ int cufinufftf_destroy(cufinufftf_plan plan);
  * when every axis has 3 bins or more, the halo ceil(ns/2) is no wider than
  * a bin and the tile no wider than the grid; ignored otherwise. */
 #define CUFINUFFT_TILE_SLOTS 2
-/* CUFINUFFT_FULL_FFT (3D): transform the whole fine grid with one 3D hipFFT
- * plan, as the reference does with cuFFT.  Without it a 3D plan runs the FFT
- * as three contiguous batched 1D passes over only the grid lines that hold
- * modes (type 1: the mode columns after the x pass, the mode rows after the
- * y pass; type 2 in reverse), with its own transposes between the passes
- * and the deconvolve / amplify folded into the first or last of them; the
- * plan then holds a further ms0 * nf1 * nf2 complex values per batch member.
- * The modes agree with the full transform's to rounding. */
+/* CUFINUFFT_FULL_FFT (2D, 3D): transform the whole fine grid with one 2D / 3D
+ * hipFFT plan, as the reference does with cuFFT.  Without it such a plan runs
+ * the FFT as contiguous batched 1D passes over only the grid lines that hold
+ * modes (type 1: the mode columns after the x pass, in 3D the mode rows
+ * after the y pass; type 2 in reverse), with its own transposes between the
+ * passes and the deconvolve / amplify folded into the first or last of them;
+ * the plan then holds a further ms0 * nf1 * nf2 complex values per batch
+ * member.  The modes agree with the full transform's to rounding. */
 #define CUFINUFFT_FULL_FFT 4
 int cufinufft_makeplan_ext(int type, int dim, int *nmodes, int iflag, int ntransf,
                            double tol, int maxbatchsize, cufinufft_plan *plan,

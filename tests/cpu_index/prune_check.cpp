@@ -1,12 +1,13 @@
-// Host check of the pruned 3D FFT's data movement (fft_prune.hip): the
+// Host check of the pruned 2D / 3D FFT's data movement (fft_prune.hip): the
 // library's transpose parameters (prune_* in nufft_launch.h) and offset maps,
 // driven element by element as the GPU transposes apply them, with naive 1D
-// DFTs in place of hipFFT, against direct 3D sums.  Every read and write is
+// DFTs in place of hipFFT, against direct 2D / 3D sums.  Every read and write is
 // bounds-checked against its buffer; every output element of a transpose must
 // be written exactly once (scatters: every cell, zeros included; gathers:
 // every mode).  Built with AddressSanitizer and UndefinedBehaviorSanitizer.
 //
-// usage: prune_check ms0 ms1 ms2 nf0 nf1 nf2 ntr iflag
+// usage: prune_check ms0 ms1 ms2 nf0 nf1 nf2 ntr iflag   (ms2 = nf2 = 1: the 2D
+// passes)
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -65,7 +66,9 @@ static void gather(const std::vector<cd> &in, std::vector<cd> &out, const PruneG
                         continue;
                     }
                     cd v = in[s];
-                    if (kr) v /= kr[std::abs(r - d.hr)] * (kp[std::abs(p - d.hp)] * kc[std::abs(q - d.kneg)]);
+                    if (kr)
+                        v /= kr[std::abs(r - d.hr)] *
+                             ((kp ? kp[std::abs(p - d.hp)] : 1.0) * kc[std::abs(q - d.kneg)]);
                     out[o] = v;
                     ++hits[o];
                 }
@@ -105,7 +108,8 @@ static void scatter(const std::vector<cd> &in, std::vector<cd> &out, const Prune
                         }
                         v = in[s];
                         if (kr) {
-                            v /= kr[std::abs(r - d.hr)] * (kp[std::abs(p - d.hp)] * kc[std::abs(q - d.kneg)]);
+                            v /= kr[std::abs(r - d.hr)] *
+                                 ((kp ? kp[std::abs(p - d.hp)] : 1.0) * kc[std::abs(q - d.kneg)]);
                             if (wsel >= 0) {
                                 const int f = wsel == 0 ? r - d.hr : (wsel == 1 ? p - d.hp : q - d.kneg);
                                 v *= cd(0, (double)(wsign * f));
@@ -169,13 +173,22 @@ int main(int argc, char **argv)
     fw0 = fw;
     std::vector<cd> b1((long long)ms[0] * nf[2] * nf[1] * ntr);
     std::vector<cd> fk(ntot * ntr);
+    const bool two = ms[2] == 1 && nf[2] == 1;
+    if (two) ker[2][0] = 1.0;  // no z factor in 2D
     dft_lines(fw, (long long)nf[1] * nf[2] * ntr, nf[0], iflag);
-    gather(fw, b1, cfa::prune_fw_to_b1(ms, nf), ntr, nullptr, nullptr, nullptr, "fw->B1");
-    dft_lines(b1, (long long)ms[0] * nf[2] * ntr, nf[1], iflag);
-    gather(b1, fw, cfa::prune_b1_to_b2(ms, nf), ntr, nullptr, nullptr, nullptr, "B1->B2");
-    dft_lines(fw, (long long)ms[0] * ms[1] * ntr, nf[2], iflag);  // B2 at the start of fw
-    gather(fw, fk, cfa::prune_b2_to_fk(ms, nf), ntr, ker[0].data(), ker[1].data(), ker[2].data(),
-           "B2->fk");
+    if (two) {
+        gather(fw, b1, cfa::prune2_fw_to_b1(ms, nf), ntr, nullptr, nullptr, nullptr, "fw->B1 2D");
+        dft_lines(b1, (long long)ms[0] * ntr, nf[1], iflag);
+        gather(b1, fk, cfa::prune2_b1_to_fk(ms, nf), ntr, ker[0].data(), nullptr, ker[1].data(),
+               "B1->fk 2D");
+    } else {
+        gather(fw, b1, cfa::prune_fw_to_b1(ms, nf), ntr, nullptr, nullptr, nullptr, "fw->B1");
+        dft_lines(b1, (long long)ms[0] * nf[2] * ntr, nf[1], iflag);
+        gather(b1, fw, cfa::prune_b1_to_b2(ms, nf), ntr, nullptr, nullptr, nullptr, "B1->B2");
+        dft_lines(fw, (long long)ms[0] * ms[1] * ntr, nf[2], iflag);  // B2 at the start of fw
+        gather(fw, fk, cfa::prune_b2_to_fk(ms, nf), ntr, ker[0].data(), ker[1].data(),
+               ker[2].data(), "B2->fk");
+    }
     std::vector<cd> ref(ntot * ntr);
     for (int t = 0; t < ntr; ++t)
         for (int kz = 0; kz < ms[2]; ++kz)
@@ -201,17 +214,25 @@ int main(int argc, char **argv)
     // ---- type 2: fk -> fw, plain and with each derivative weight ----------
     std::vector<cd> fkin(ntot * ntr);
     for (auto &v : fkin) v = cd(nd(rng), nd(rng));
-    for (int wsel = -1; wsel < 3; ++wsel) {
+    for (int wsel = -1; wsel < (two ? 2 : 3); ++wsel) {
         const int wsign = iflag;
         std::vector<cd> g(nftot * ntr, cd(99, 99)), h(b1.size(), cd(99, 99));
-        scatter(fkin, g, cfa::prune_fk_to_b2(ms, nf), ntr, ker[0].data(), ker[1].data(),
-                ker[2].data(), wsel, wsign, "fk->B2", false);
-        dft_lines(g, (long long)ms[0] * ms[1] * ntr, nf[2], iflag);
-        scatter(g, h, cfa::prune_b2_to_b1(ms, nf), ntr, nullptr, nullptr, nullptr, -1, 1,
-                "B2->B1", true);
-        dft_lines(h, (long long)ms[0] * nf[2] * ntr, nf[1], iflag);
-        scatter(h, g, cfa::prune_b1_to_fw(ms, nf), ntr, nullptr, nullptr, nullptr, -1, 1,
-                "B1->fw", true);
+        if (two) {  // the library maps axis y to the scatter's c axis (wsel 2)
+            scatter(fkin, h, cfa::prune2_fk_to_b1(ms, nf), ntr, ker[0].data(), nullptr,
+                    ker[1].data(), wsel == 1 ? 2 : wsel, wsign, "fk->B1 2D", true);
+            dft_lines(h, (long long)ms[0] * ntr, nf[1], iflag);
+            scatter(h, g, cfa::prune2_b1_to_fw(ms, nf), ntr, nullptr, nullptr, nullptr, -1, 1,
+                    "B1->fw 2D", true);
+        } else {
+            scatter(fkin, g, cfa::prune_fk_to_b2(ms, nf), ntr, ker[0].data(), ker[1].data(),
+                    ker[2].data(), wsel, wsign, "fk->B2", false);
+            dft_lines(g, (long long)ms[0] * ms[1] * ntr, nf[2], iflag);
+            scatter(g, h, cfa::prune_b2_to_b1(ms, nf), ntr, nullptr, nullptr, nullptr, -1, 1,
+                    "B2->B1", true);
+            dft_lines(h, (long long)ms[0] * nf[2] * ntr, nf[1], iflag);
+            scatter(h, g, cfa::prune_b1_to_fw(ms, nf), ntr, nullptr, nullptr, nullptr, -1, 1,
+                    "B1->fw", true);
+        }
         dft_lines(g, (long long)nf[1] * nf[2] * ntr, nf[0], iflag);
         std::vector<cd> r2(nftot * ntr);
         for (int t = 0; t < ntr; ++t)

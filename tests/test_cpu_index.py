@@ -68,16 +68,20 @@ PRUNE_CASES = [
     "8 3 1 8 6 4 2 1",       # modes filling an axis (ms = nf), one mode on another
     "7 7 6 16 15 12 1 1",
     "9 8 8 12 10 10 3 -1",   # sigma 1.25-like grids: few non-mode cells
+    # 2D (ms2 = nf2 = 1)
+    "6 5 1 12 10 1 2 1",
+    "7 4 1 16 9 1 3 -1",
+    "8 8 1 8 10 1 1 1",      # modes filling the x axis
 ]
 
 
 @pytest.mark.parametrize("case", PRUNE_CASES)
 def test_pruned_fft_moves(prune_check, case):
-    """The pruned 3D FFT's transposes (the library's own parameter builders and
-    offset maps), applied element by element with naive DFTs in place of
-    hipFFT: type 1 modes and type 2 grids (plain and with each derivative
-    weight) match direct 3D sums to 1e-12; every buffer access in bounds and
-    every output element written once."""
+    """The pruned 2D / 3D FFT's transposes (the library's own parameter
+    builders and offset maps), applied element by element with naive DFTs in
+    place of hipFFT: type 1 modes and type 2 grids (plain and with each
+    derivative weight) match direct sums to 1e-12; every buffer access in
+    bounds and every output element written once."""
     r = subprocess.run([prune_check] + case.split(), capture_output=True, text=True,
                        timeout=300)
     print(r.stdout)

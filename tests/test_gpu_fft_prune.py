@@ -1,16 +1,18 @@
-"""The pruned 3D FFT against the one-plan 3D FFT (full_fft=True, the
+"""The pruned 2D / 3D FFT against the one-plan FFT (full_fft=True, the
 reference's cuFFT structure) on the same points and inputs.
 
 Both run the same spread / interp; only the FFT stage differs (three
 contiguous 1D passes over the grid lines that hold modes, with the
 deconvolve / amplify folded into the transposes, against one 3D hipFFT plus
-the separate deconvolve / amplify).  Their outputs agree to rounding: bars
-1e-12 (FP64) and 1e-6 (FP32) relative l2, three orders below the transforms'
-tolerances.  Accuracy against the oracle and direct sums is covered by the
-3D cases of the parity tests, which run the pruned path (the default).
-Cases: odd and even mode counts, sigma 1.25, a batch split into a full and a
-partial FFT batch (ntransf 3, maxbatchsize 2), execute_grad's derivative
-modes, and a 3D plan group."""
+the separate deconvolve / amplify).  Their outputs agree to rounding: bars 1e-12 (FP64)
+relative l2, and for FP32 1e-6 or, where larger, twice the full-FFT FP32
+result's own distance from the FP64 pipeline on the same inputs (same kernel
+width): both are FP32 FFTs of one grid, and at sigma 1.25 the deconvolution
+multiplies FFT rounding by up to ~10 at the edge modes.  Accuracy against the oracle and direct sums is covered by the
+2D and 3D cases of the parity tests, which run the pruned path (the
+default).  Cases: odd and even mode counts, sigma 1.25, a batch split into a
+full and a partial FFT batch (ntransf 3, maxbatchsize 2), execute_grad's
+derivative modes, and 3D plan groups (2D groups: the group tests)."""
 import numpy as np
 import pytest
 
@@ -25,60 +27,83 @@ CASES = [
     (np.float32, (32, 30, 28), 40_000, 1e-5, 1, {}),
     (np.float64, (21, 16, 18), 10_000, 1e-6, 1, dict(upsampfac=1.25)),
     (np.float32, (40, 9, 33), 30_000, 1e-4, 2, dict(gpu_method=1, maxbatchsize=2)),
+    (np.float32, (64, 48), 20_000, 1e-6, 3, dict(maxbatchsize=2)),
+    (np.float64, (70, 33), 10_000, 1e-9, 1, {}),
+    (np.float32, (45, 64), 10_000, 1e-5, 1, dict(upsampfac=1.25)),
 ]
 
 
-def _bar(dtype):
-    return 1e-12 if dtype == np.float64 else 1e-6
+def _bar(dtype, full=None, ref64=None):
+    if dtype == np.float64:
+        return 1e-12
+    return max(1e-6, 2 * rel_l2(full, ref64)) if ref64 is not None else 1e-6
+
+
+def _as64(a):
+    return a.astype(np.complex128 if np.iscomplexobj(a) else np.float64)
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"{np.dtype(c[0]).name}-{c[1]}-t{c[4]}")
 def test_type1_pruned_vs_full(torch_dev, case):
     dtype, nm, M, tol, ntr, opts = case
     print(f"pruned vs full type 1 {np.dtype(dtype).name} {nm} ntr {ntr} {opts}", flush=True)
-    pts = G.clustered_pts(M, 3, dtype, seed=5)
+    pts = G.clustered_pts(M, len(nm), dtype, seed=5)
     c = G.strengths(M, dtype, seed=5, ntr=ntr)
     a, pa = G.run_type1(torch_dev, pts, c, nm, tol, dtype, ntr=ntr, **opts)
     b, pb = G.run_type1(torch_dev, pts, c, nm, tol, dtype, ntr=ntr, full_fft=True, **opts)
     assert pa.info()["nf"] == pb.info()["nf"]
+    ref64 = None
+    if dtype == np.float32:
+        ref64, p64 = G.run_type1(torch_dev, [_as64(p) for p in pts], _as64(c), nm, tol,
+                                 np.float64, ntr=ntr, full_fft=True, **opts)
+        assert p64.info()["ns"] == pb.info()["ns"]
     err = rel_l2(a, b)
-    print(f"   rel-l2 {err:.3e}", flush=True)
-    assert err <= _bar(dtype)
+    bar = _bar(dtype, b, ref64)
+    print(f"   rel-l2 {err:.3e} (bar {bar:.3e})", flush=True)
+    assert err <= bar
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"{np.dtype(c[0]).name}-{c[1]}-t{c[4]}")
 def test_type2_pruned_vs_full(torch_dev, case):
     dtype, nm, M, tol, ntr, opts = case
     print(f"pruned vs full type 2 {np.dtype(dtype).name} {nm} ntr {ntr} {opts}", flush=True)
-    pts = G.uniform_pts(M, 3, dtype, seed=6)
+    pts = G.uniform_pts(M, len(nm), dtype, seed=6)
     fk = G.modes(tuple(nm[::-1]), dtype, seed=6, ntr=ntr)
     a, _ = G.run_type2(torch_dev, pts, fk, tol, dtype, ntr=ntr, **opts)
-    b, _ = G.run_type2(torch_dev, pts, fk, tol, dtype, ntr=ntr, full_fft=True, **opts)
+    b, pb = G.run_type2(torch_dev, pts, fk, tol, dtype, ntr=ntr, full_fft=True, **opts)
+    ref64 = None
+    if dtype == np.float32:
+        ref64, p64 = G.run_type2(torch_dev, [_as64(p) for p in pts], _as64(fk), tol, np.float64,
+                                 ntr=ntr, full_fft=True, **opts)
+        assert p64.info()["ns"] == pb.info()["ns"]
     err = rel_l2(a, b)
-    print(f"   rel-l2 {err:.3e}", flush=True)
-    assert err <= _bar(dtype)
+    bar = _bar(dtype, b, ref64)
+    print(f"   rel-l2 {err:.3e} (bar {bar:.3e})", flush=True)
+    assert err <= bar
 
 
-@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-9), (np.float32, 1e-5)])
-def test_execute_grad_pruned_vs_full(torch_dev, dtype, tol):
+@pytest.mark.parametrize("dtype,tol,nm", [(np.float64, 1e-9, (20, 18, 15)),
+                                          (np.float32, 1e-5, (20, 18, 15)),
+                                          (np.float32, 1e-6, (40, 36))])
+def test_execute_grad_pruned_vs_full(torch_dev, dtype, tol, nm):
     """The derivative modes (i * isign * k_d weights in the first scatter)."""
     import torch
     from cufinufft_amd import cufinufft
-    nm, M, ntr = (20, 18, 15), 8000, 2
-    print(f"pruned vs full execute_grad {np.dtype(dtype).name}", flush=True)
-    pts = G.uniform_pts(M, 3, dtype, seed=9)
+    M, ntr, dim = 8000, 2, len(nm)
+    print(f"pruned vs full execute_grad {np.dtype(dtype).name} {nm}", flush=True)
+    pts = G.uniform_pts(M, dim, dtype, seed=9)
     fk = G.to_dev(G.modes(tuple(nm[::-1]), dtype, seed=9, ntr=ntr), torch_dev)
     out = []
     for full in (False, True):
         plan = cufinufft(2, nm[::-1], n_trans=ntr, eps=tol, dtype=dtype, full_fft=full)
         plan.set_pts(*[G.to_dev(p, torch_dev) for p in pts[::-1]])
         c = torch.zeros(M * ntr, dtype=fk.dtype, device=torch_dev)
-        dc = torch.zeros((3, M * ntr), dtype=fk.dtype, device=torch_dev)
+        dc = torch.zeros((dim, M * ntr), dtype=fk.dtype, device=torch_dev)
         plan.execute_grad(c, dc, fk)
         torch.cuda.synchronize()
         out.append((c.cpu().numpy(), dc.cpu().numpy()))
     assert rel_l2(out[0][0], out[1][0]) <= _bar(dtype)
-    for d in range(3):
+    for d in range(dim):
         err = rel_l2(out[0][1][d], out[1][1][d])
         print(f"   d/dx{d} rel-l2 {err:.3e}", flush=True)
         assert err <= _bar(dtype)
