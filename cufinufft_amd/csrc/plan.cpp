@@ -922,10 +922,11 @@ int Plan<T>::makeplan(int type_, int dim_, const int *nmodes, int iflag_, int nt
                 return kErrHip;
         }
         if (dmalloc(&fw, (size_t)(nftot * maxbatch))) return kErrHip;
-        // 3D grids take the pruned FFT unless the caller asks for the full
-        // one; its launch grids and hipFFT batch counts bound the sizes.
+        // 2D and 3D grids take the pruned FFT unless the caller asks for the
+        // full one; the transposes' plane counts (grid y: nf2, N1, N2 in 3D)
+        // and batch (grid z) and hipFFT's int batch counts bound the sizes.
         pruned = dim >= 2 && !(flags & CUFINUFFT_FULL_FFT) && maxbatch <= 65535 &&
-                 nf[1] <= 65535 && nf[2] <= 65535 &&
+                 nf[0] <= 65535 && nf[1] <= 65535 && nf[2] <= 65535 &&
                  (long long)nf[1] * nf[2] * maxbatch <= 2147483647LL &&
                  (long long)ms[0] * nf[2] * nf[1] * maxbatch <= 2147483647LL;
         if (pruned) {
