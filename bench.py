@@ -308,10 +308,10 @@ def check_accuracy(w, out, sel, ref):
 
 # -------------------------------------------------------- CPU baseline
 def cpu_baseline(w, pts, host_in, budget_pts):
-    """The same pipeline on the host cores (oracle/: FP64, OpenMP
-    interp/spread, scipy FFT with all workers) on a bounded sample of the
-    points; the fixed grid cost (amplify/deconvolve + FFT) is timed at full
-    size and the per-point cost is extrapolated to M."""
+    """The same pipeline on the host cores (oracle/: FP64, OpenMP interp and
+    slab-parallel spread, scipy FFT with all workers) on a bounded sample of
+    the points; the fixed grid cost (amplify/deconvolve + FFT) is timed at
+    full size and the per-point cost is extrapolated to M."""
     from oracle import oracle as O
     single = w["dtype"] == "float32"
     cores = O.num_threads()
@@ -334,16 +334,15 @@ def cpu_baseline(w, pts, host_in, budget_pts):
     if w["type"] == 2:
         O.interp(sub, fw, P.nf, P.sp)
     else:
-        O.spread(sub, host_in[:Ms], P.nf, P.sp)
+        O.spread(sub, host_in[:Ms], P.nf, P.sp, threads=True)
     t_pts = time.perf_counter() - t0
     t_full = t_fixed + t_pts * (w["M"] / Ms)
     T = w["ntransf"]
     value = w["M"] * T / (t_full * T)
-    threads = cores if w["type"] == 2 else 1
-    return dict(value=value, unit="NU pts/s", cores=threads, kind="port",
+    return dict(value=value, unit="NU pts/s", cores=cores, kind="port",
                 sample=(f"oracle/ FP64 pipeline: grid stages (amplify/deconvolve + scipy FFT, "
                         f"{cores} workers) timed at full size ({t_fixed:.2f} s), "
-                        f"{'interp (OpenMP)' if w['type'] == 2 else 'spread (1 thread)'} timed on "
+                        f"{'interp' if w['type'] == 2 else 'spread'} ({cores} OpenMP threads) timed on "
                         f"{Ms} of {w['M']} points ({t_pts:.2f} s) and extrapolated; "
                         f"one transform"))
 
@@ -654,7 +653,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--no-fp64", action="store_true",
                     help="c3: skip the FP64 (tol 1e-9) pipeline on the same points")
-    ap.add_argument("--cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=10_000_000)
     ap.add_argument("--opts", default=None,
                     help="JSON dict of cufinufft_opts fields for the tuned plan (skips the sweep)")
     ap.add_argument("--copy-calib", type=int, default=0,

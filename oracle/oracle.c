@@ -304,6 +304,89 @@ void orc_spread(int dim, int M, const double *xr, const double *yr,
     }
 }
 
+/* The same spread on all OpenMP threads, for the CPU baseline (the bench
+   times it; the checks use orc_spread): the last axis of the grid is cut
+   into one slab of planes per thread, points are bucketed by their first
+   tap plane on that axis (counting sort), and each thread visits the
+   buckets whose taps can reach its slab and adds only the taps that land in
+   it -- every grid cell has one writer, so no atomics.  Points near a slab
+   boundary have their kernel values computed by two threads.  The sum per
+   cell is orc_spread's, in another order.                                  */
+void orc_spread_omp(int dim, int M, const double *xr, const double *yr,
+                    const double *zr, const double *c, const int *nf, int ns,
+                    double beta, double es_c, double *fw)
+{
+    const int n1 = nf[0], n2 = dim > 1 ? nf[1] : 1, n3 = dim > 2 ? nf[2] : 1;
+    const double *ax = dim == 3 ? zr : (dim == 2 ? yr : xr);  /* slab axis */
+    const int n = dim == 3 ? n3 : (dim == 2 ? n2 : n1);
+    memset(fw, 0, sizeof(double) * 2 * (size_t)n1 * n2 * n3);
+    int *start = (int *)calloc((size_t)n + 1, sizeof(int));
+    int *order = (int *)malloc(sizeof(int) * (size_t)(M > 0 ? M : 1));
+    int *pl = (int *)malloc(sizeof(int) * (size_t)(M > 0 ? M : 1));
+    if (!start || !order || !pl) {
+        free(start);
+        free(order);
+        free(pl);
+        orc_spread(dim, M, xr, yr, zr, c, nf, ns, beta, es_c, fw);
+        return;
+    }
+    double kdummy[ORC_MAX_NSPREAD];
+    for (int j = 0; j < M; ++j) {
+        pl[j] = wrapi(kervec(ax[j], ns, beta, es_c, kdummy), n);
+        start[pl[j] + 1]++;
+    }
+    for (int p = 0; p < n; ++p) start[p + 1] += start[p];
+    {
+        int *fill = (int *)malloc(sizeof(int) * (size_t)n);
+        memcpy(fill, start, sizeof(int) * (size_t)n);
+        for (int j = 0; j < M; ++j) order[fill[pl[j]]++] = j;
+        free(fill);
+    }
+#pragma omp parallel
+    {
+        const int nt = omp_get_num_threads(), t = omp_get_thread_num();
+        const int a = (int)((long long)n * t / nt), b = (int)((long long)n * (t + 1) / nt);
+        double k1[ORC_MAX_NSPREAD], k2[ORC_MAX_NSPREAD], k3[ORC_MAX_NSPREAD];
+        /* buckets whose taps reach [a, b): first planes a-ns+1 .. b-1, each once */
+        int span = b - a + ns - 1;
+        if (span > n) span = n;
+        for (int q = 0; q < (b > a ? span : 0); ++q) {
+            const int p = wrapi(a - ns + 1 + q, n);
+            for (int e = start[p]; e < start[p + 1]; ++e) {
+                const int j = order[e];
+                int s1 = kervec(xr[j], ns, beta, es_c, k1), s2 = 0, s3 = 0;
+                int w2 = 1, w3 = 1;
+                k2[0] = k3[0] = 1.0;
+                if (dim > 1) { s2 = kervec(yr[j], ns, beta, es_c, k2); w2 = ns; }
+                if (dim > 2) { s3 = kervec(zr[j], ns, beta, es_c, k3); w3 = ns; }
+                const double cr = c[2 * j], ci = c[2 * j + 1];
+                for (int c3 = 0; c3 < w3; ++c3) {
+                    const int l3 = wrapi(s3 + c3, n3);
+                    if (dim == 3 && (l3 < a || l3 >= b)) continue;
+                    const size_t o3 = (size_t)l3 * n2;
+                    for (int c2 = 0; c2 < w2; ++c2) {
+                        const int l2 = wrapi(s2 + c2, n2);
+                        if (dim == 2 && (l2 < a || l2 >= b)) continue;
+                        const size_t o2 = (o3 + l2) * n1;
+                        const double k23 = k2[c2] * k3[c3];
+                        for (int c1 = 0; c1 < ns; ++c1) {
+                            const int l1 = wrapi(s1 + c1, n1);
+                            if (dim == 1 && (l1 < a || l1 >= b)) continue;
+                            const size_t o = o2 + l1;
+                            const double kv = k1[c1] * k23;
+                            fw[2 * o] += cr * kv;
+                            fw[2 * o + 1] += ci * kv;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    free(start);
+    free(order);
+    free(pl);
+}
+
 /* Interp (type-2 step 3): c_j = sum_l fw[l] prod_d phi(l_d - xr_jd);
    Interp_{2,3}d_NUptsdriven (spreadinterp2d.cu:516-554,
    spreadinterp3d.cu:654-705).  Points are independent, so OpenMP splits

@@ -63,6 +63,7 @@ def _declare(L):
     L.orc_es_kernel.argtypes = [d, i, d, d]
     L.orc_es_kernel.restype = d
     L.orc_spread.argtypes = [i, i, p, p, p, p, p, i, d, d, p]
+    L.orc_spread_omp.argtypes = [i, i, p, p, p, p, p, i, d, d, p]
     L.orc_interp.argtypes = [i, i, p, p, p, p, p, i, d, d, p]
     L.orc_gauss_legendre.argtypes = [i, p, p]
     L.orc_fseries.argtypes = [i, i, d, d, p]
@@ -200,15 +201,17 @@ def _rescaled(coords, nf, exact=True):
     return [rescale(c, n).astype(np.float64) for c, n in zip(coords, nf)]
 
 
-def spread(coords, c, nf, sp, exact=True):
-    """Spread strengths c (complex, length M) to the fine grid (shape nf[::-1])."""
+def spread(coords, c, nf, sp, exact=True, threads=False):
+    """Spread strengths c (complex, length M) to the fine grid (shape nf[::-1]).
+    threads=True: the OpenMP slab spread (the CPU baseline's; same sums)."""
     dim = len(coords)
     xr = _rescaled(coords, nf, exact) + [None] * (3 - dim)
     cc = _c(c, np.complex128)
     nfa = np.array(list(nf) + [1] * (3 - dim), dtype=np.int32)
     fw = np.empty(tuple(nf[::-1]), np.complex128)
-    lib().orc_spread(dim, cc.size, _ptr(xr[0]), _ptr(xr[1]), _ptr(xr[2]), _ptr(cc), _ptr(nfa),
-                     int(sp.ns), float(sp.beta), float(sp.c), _ptr(fw))
+    fn = lib().orc_spread_omp if threads else lib().orc_spread
+    fn(dim, cc.size, _ptr(xr[0]), _ptr(xr[1]), _ptr(xr[2]), _ptr(cc), _ptr(nfa),
+       int(sp.ns), float(sp.beta), float(sp.c), _ptr(fw))
     return fw
 
 

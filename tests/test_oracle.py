@@ -316,3 +316,23 @@ def test_reference_fp32_coordinate_floor():
     e_ref = rel_l2(O.Plan(1, list(N), 1e-6, True, exact=False).type1(pts, c).ravel()[kidx], ref)
     e_lib = rel_l2(O.Plan(1, list(N), 1e-6, True, exact=True).type1(pts, c).ravel()[kidx], ref)
     assert e_ref > 5e-6 and e_lib < 2e-6
+
+
+@pytest.mark.parametrize("dim,nm,M", [(1, (200,), 3000), (2, (40, 36), 8000),
+                                      (3, (20, 18, 16), 8000), (3, (6, 6, 6), 500),
+                                      (2, (24, 8), 0)])
+def test_spread_threads_matches_sequential(dim, nm, M):
+    """The OpenMP slab spread (the CPU baseline's) against the sequential
+    spread the checks use: the same per-cell sums in another order, so equal
+    to rounding -- including grids with fewer planes per thread than taps
+    (6^3 modes: 12 planes for the threads) and no points."""
+    rng = np.random.default_rng(dim + M)
+    P = O.Plan(1, list(nm), 1e-6, single=False)
+    pts = [rng.uniform(-np.pi, np.pi, M) for _ in range(dim)]
+    c = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    a = O.spread(pts, c, P.nf, P.sp)
+    b = O.spread(pts, c, P.nf, P.sp, threads=True)
+    if M == 0:
+        assert not a.any() and not b.any()
+    else:
+        assert rel_l2(b, a) < 1e-13
