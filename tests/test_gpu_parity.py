@@ -420,6 +420,45 @@ def test_type1_fp32_fixed_point_width(torch_dev, maxsub):
     assert err <= _bar(tol, oerr)
 
 
+MULTI_CASES = [
+    # dim, dtype, nmodes, M, tol, bins, maxsub, distribution
+    (2, np.float32, (96, 80), 60_000, 1e-5, (16, 16), 4096, "clustered"),  # 32-bit cells, drains
+    (2, np.float64, (64, 48), 20_000, 1e-9, (16, 16), 1024, "uniform"),    # 64-bit cells
+    (1, np.float32, (3000,), 40_000, 1e-5, (256,), 2048, "clustered"),
+]
+
+
+@pytest.mark.parametrize("case", MULTI_CASES, ids=lambda c: f"{c[0]}d-{np.dtype(c[1]).name}")
+def test_type1_batched_scales(torch_dev, case):
+    """1D / 2D type 1 with 7 transforms in one batch (one spread launch over
+    all of them).  Transform t's strengths are scaled by 10^(2t-6), so a
+    fixed-point scale shared between transforms would fail the small ones.
+    Each transform against the oracle at tol / 5 (relative l2), and against
+    the same transform planned alone: the same subproblems, chunks and scales
+    with exact integer tile sums, so equal to FFT rounding (1e-6 FP32,
+    1e-13 FP64)."""
+    dim, dtype, nm, M, tol, bins, maxsub, dist = case
+    ntr = 7
+    _flush(f"batched scales {dim}d {np.dtype(dtype).name} ntr {ntr}")
+    pts = (G.uniform_pts if dist == "uniform" else G.clustered_pts)(M, dim, dtype, seed=44)
+    c = G.strengths(M, dtype, seed=44, ntr=ntr)
+    for t in range(ntr):
+        c[t * M:(t + 1) * M] *= 10.0 ** (2 * t - 6)
+    opts = dict(gpu_method=2, gpu_maxsubprobsize=maxsub, gpu_binsizex=bins[0])
+    if dim > 1:
+        opts["gpu_binsizey"] = bins[1]
+    fk, _ = G.run_type1(torch_dev, pts, c, nm, tol, dtype, ntr=ntr, maxbatchsize=ntr, **opts)
+    P = O.Plan(1, list(nm), tol, single=dtype == np.float32)
+    for t in range(ntr):
+        ref = P.type1(pts, c[t * M:(t + 1) * M])
+        err = rel_l2(fk[t], ref)
+        one, _ = G.run_type1(torch_dev, pts, c[t * M:(t + 1) * M], nm, tol, dtype, **opts)
+        d1 = rel_l2(fk[t], one)
+        _flush(f"   transform {t}: vs oracle {err:.3e}, vs alone {d1:.3e}")
+        assert err <= tol / 5, (t, err)
+        assert d1 <= (1e-6 if dtype == np.float32 else 1e-13), (t, d1)
+
+
 def test_many_batched(torch_dev):
     """ntransf=5 through maxbatchsize=2 (a partial last batch), type 1 and 2."""
     dtype, nm, M, tol, ntr = np.float32, (48, 40), 8000, 1e-5, 5
