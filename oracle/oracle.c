@@ -321,10 +321,12 @@ void orc_spread_omp(int dim, int M, const double *xr, const double *yr,
     const int n = dim == 3 ? n3 : (dim == 2 ? n2 : n1);
     memset(fw, 0, sizeof(double) * 2 * (size_t)n1 * n2 * n3);
     int *start = (int *)calloc((size_t)n + 1, sizeof(int));
+    int *fill = (int *)malloc(sizeof(int) * (size_t)n);
     int *order = (int *)malloc(sizeof(int) * (size_t)(M > 0 ? M : 1));
     int *pl = (int *)malloc(sizeof(int) * (size_t)(M > 0 ? M : 1));
-    if (!start || !order || !pl) {
+    if (!start || !fill || !order || !pl) {
         free(start);
+        free(fill);
         free(order);
         free(pl);
         orc_spread(dim, M, xr, yr, zr, c, nf, ns, beta, es_c, fw);
@@ -336,12 +338,9 @@ void orc_spread_omp(int dim, int M, const double *xr, const double *yr,
         start[pl[j] + 1]++;
     }
     for (int p = 0; p < n; ++p) start[p + 1] += start[p];
-    {
-        int *fill = (int *)malloc(sizeof(int) * (size_t)n);
-        memcpy(fill, start, sizeof(int) * (size_t)n);
-        for (int j = 0; j < M; ++j) order[fill[pl[j]]++] = j;
-        free(fill);
-    }
+    memcpy(fill, start, sizeof(int) * (size_t)n);
+    for (int j = 0; j < M; ++j) order[fill[pl[j]]++] = j;
+    free(fill);
 #pragma omp parallel
     {
         const int nt = omp_get_num_threads(), t = omp_get_thread_num();
