@@ -103,20 +103,35 @@ k_sort_hist(int M, const T *__restrict__ x, const T *__restrict__ y, const T *__
     for (int d = threadIdx.x; d < nd; d += 256) h[d] = 0;
     __syncthreads();
     const long long base = (long long)blockIdx.x * kSortTile;
-    for (int k = 0; k < kSortTile / 256; ++k) {
-        const long long j = base + k * 256 + threadIdx.x;
-        if (j >= M) break;
-        int key;
-        if constexpr (FIRST) {
-            T v[3];
-            v[0] = x[j];
-            if (DIM > 1) v[1] = y[j];
-            if (DIM > 2) v[2] = z[j];
-            key = sort_key<T, DIM, FINE>(v, g);
-        } else {
-            key = kin[j];
+    // all of a thread's loads in flight before the first wait, then the keys
+    // and their counts
+    constexpr int kPer = kSortTile / 256;
+    if constexpr (FIRST) {
+        T v[kPer][3];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const long long j = base + k * 256 + threadIdx.x;
+            v[k][0] = v[k][1] = v[k][2] = T(0);
+            if (j < M) {
+                v[k][0] = x[j];
+                if (DIM > 1) v[k][1] = y[j];
+                if (DIM > 2) v[k][2] = z[j];
+            }
         }
-        atomicAdd(&h[(key >> shift) & (nd - 1)], 1);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+            if (base + k * 256 + threadIdx.x < M)
+                atomicAdd(&h[(sort_key<T, DIM, FINE>(v[k], g) >> shift) & (nd - 1)], 1);
+    } else {
+        int key[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const long long j = base + k * 256 + threadIdx.x;
+            key[k] = j < M ? kin[j] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+            if (base + k * 256 + threadIdx.x < M) atomicAdd(&h[(key[k] >> shift) & (nd - 1)], 1);
     }
     __syncthreads();
     for (int d = threadIdx.x; d < nd; d += 256) counts[(long long)d * tiles + blockIdx.x] = h[d];
@@ -159,22 +174,35 @@ k_sort_scatter(int M, const T *__restrict__ x, const T *__restrict__ y, const T 
     const long long tbase = (long long)blockIdx.x * kSortTile;
     const long long wbase = tbase + w * (kSortRounds * 64);
     int keys[kSortRounds];  // this lane's keys, kept for the ranking rounds
+    // all of the lane's loads in flight before the first wait, then the keys
+    // and their counts
+    if constexpr (FIRST) {
+        T v[kSortRounds][3];
+#pragma unroll
+        for (int r = 0; r < kSortRounds; ++r) {
+            const long long j = wbase + r * 64 + lane;
+            v[r][0] = v[r][1] = v[r][2] = T(0);
+            if (j < M) {
+                v[r][0] = x[j];
+                if (DIM > 1) v[r][1] = y[j];
+                if (DIM > 2) v[r][2] = z[j];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kSortRounds; ++r) keys[r] = sort_key<T, DIM, FINE>(v[r], g);
+    } else {
+#pragma unroll
+        for (int r = 0; r < kSortRounds; ++r) {
+            const long long j = wbase + r * 64 + lane;
+            keys[r] = j < M ? kin[j] : 0;
+        }
+    }
 #pragma unroll
     for (int r = 0; r < kSortRounds; ++r) {
-        const long long j = wbase + r * 64 + lane;
-        keys[r] = 0;
-        if (j < M) {
-            if constexpr (FIRST) {
-                T v[3];
-                v[0] = x[j];
-                if (DIM > 1) v[1] = y[j];
-                if (DIM > 2) v[2] = z[j];
-                keys[r] = sort_key<T, DIM, FINE>(v, g);
-            } else {
-                keys[r] = kin[j];
-            }
+        if (wbase + r * 64 + lane < M)
             atomicAdd(&wb[w][(keys[r] >> shift) & (nd - 1)], 1);
-        }
+        else
+            keys[r] = 0;  // items past M are never ranked or written
     }
     __syncthreads();
     // digit totals of the tile (thread t owns digits kPer t .. kPer t + kPer-1),
